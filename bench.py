@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X BCSR x dense engine on one GPU.
+
+Default workload: A = 16384 x 16384 fp16 BCSR with 16 x 16 blocks, 5% of the
+blocks nonzero ("blockrandom", seed 123), B = 16384 x 512 fp16 (seed 456),
+C fp32 -- the shape the headline metric is quoted on.  A step is one
+C = A . B product (one kernel launch) with A, B and C resident in HBM.
+
+Prints one JSON line: effective TFLOPS = 2 * nnzb * bh * bw * N / t, the
+dominant kernel's roofline position (algorithmic bytes / time vs 8 TB/s
+HBM) and the oracle's CPU product timed on this host as cpu_baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BCSR SpMM effective TFLOPS + % roofline; FP8/batched/learned vs plain FP16"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0, "hilo": 2500.0}
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--M", type=int, default=16384)
+    p.add_argument("--K", type=int, default=16384)
+    p.add_argument("--N", type=int, default=512)
+    p.add_argument("--density", type=float, default=0.05, help="fraction of nonzero blocks")
+    p.add_argument("--bh", type=int, default=16)
+    p.add_argument("--pattern", default=None,
+                   help="generator pattern (default: blockrandom / pattern32by16 / pattern64by16 by bh)")
+    p.add_argument("--tile-n", type=int, default=0)
+    p.add_argument("--xcd-map", type=int, default=-1)
+    p.add_argument("--precision", default="f16", choices=["f16"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
+    return p.parse_args()
+
+
+def algorithmic_bytes(nnzb, bh, bw, M, K, N, a_bytes=2):
+    """A values + idx + hdr + B + C, each read or written once."""
+    return nnzb * bh * bw * a_bytes + 4 * nnzb + 4 * (M // bh + 1) + K * N * 2 + M * N * 4
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+
+    import matrixmultiplicationcuda_amd as mm
+
+    pattern = args.pattern or {16: "blockrandom", 32: "pattern32by16",
+                               64: "pattern64by16"}[args.bh]
+    bh, bw = args.bh, 16
+    M, K, N = args.M, args.K, args.N
+    t0 = time.time()
+    A16 = mm.generate(M, K, 1.0 - args.density, pattern, bh=bh, bw=bh, seed=123)
+    B16 = mm.generate(K, N, 0.0, "random", bh=bh, seed=456)
+    h = mm.bcsr_from_dense(A16, bh, bw)
+    del A16
+    mm.check(mm.lib().bcsr_set_device(0))
+    plan = mm.Plan(h, args.precision)
+    t_setup = time.time() - t0
+    print(f"# setup {t_setup:.1f}s nnzb={h.nnzb}", file=sys.stderr, flush=True)
+
+    B = mm.DeviceArray.from_numpy(B16)
+    C = mm.DeviceArray((M, N), np.float32)
+    cfg = mm.make_cfg(tile_n=args.tile_n, xcd_map=args.xcd_map)
+    kname = plan.kernel_name(N, cfg)
+    stream = mm.Stream()
+    plan.spmm_repeat_ptr(B.ptr, C.ptr, N, args.warmup, stream, cfg)
+    stream.synchronize()
+    mm.synchronize()
+    start, end = mm.Event(), mm.Event()
+    start.record(stream)
+    plan.spmm_repeat_ptr(B.ptr, C.ptr, N, args.steps, stream, cfg)
+    end.record(stream)
+    stream.synchronize()
+    mm.synchronize()
+    ms = start.elapsed_ms(end) / max(args.steps, 1)
+    t = ms * 1e-3
+    print(f"# timed {args.steps} steps: {ms * 1e3:.2f} us/step ({kname})", file=sys.stderr,
+          flush=True)
+
+    flops = 2.0 * h.nnzb * bh * bw * N
+    q = algorithmic_bytes(h.nnzb, bh, bw, M, K, N)
+    tflops = flops / t / 1e12
+    gbs = q / t / 1e9
+    t_star = max(q / (HBM_PEAK_GBS * 1e9), flops / (MFMA_PEAK_TFLOPS[args.precision] * 1e12))
+    bound = "hbm" if q / (HBM_PEAK_GBS * 1e9) >= flops / (MFMA_PEAK_TFLOPS[args.precision] * 1e12) else "mfma"
+    if bound == "hbm":
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}
+    else:
+        pk = MFMA_PEAK_TFLOPS[args.precision]
+        roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": pk, "unit": "TFLOP/s",
+                "frac": round(tflops / pk, 4), "traffic": None}
+    roof["kernel"] = kname
+    roof["roofline_time_us"] = round(t_star * 1e6, 3)
+    roof["algorithmic_bytes"] = int(q)
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    wl_key = f"{M}x{K}_bh{bh}_d{args.density}_N{N}_{args.precision}"
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f).get(wl_key)
+        if tr and tr.get("kernel") == kname:
+            roof["traffic"] = tr["bytes_per_launch"]
+
+    # correctness of this very run against the CPU reference product
+    C_gpu = C.numpy()
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        Bf = B16.astype(np.float32)
+        data16 = h.data.view(np.uint16)
+        times = []
+        C_cpu = None
+        for rep in range(3):
+            c0 = time.perf_counter()
+            C_cpu = oracle.bcsr_spmm_f32(h.hdr, h.idx, data16, bh, bw, Bf, threads=threads, C=C_cpu)
+            times.append(time.perf_counter() - c0)
+            if sum(times) > 20.0:
+                break
+        tc = float(np.median(times))
+        _, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data.astype(np.float32), bh, bw, Bf,
+                                       threads=threads)
+        err = np.abs(C_gpu.astype(np.float64) - C_cpu)
+        nz = Cabs > 0
+        rel = float((err[nz] / Cabs[nz]).max(initial=0.0))
+        exact_zero = bool(np.all(C_gpu[~nz] == 0))
+        cpu = {"value": round(flops / tc / 1e12, 5), "unit": "TFLOPS", "cores": threads,
+               "kind": "port",
+               "sample": f"full product (all {h.block_rows} block-rows), oracle BCSR-order fp32 "
+                         f"loop over {threads} threads, median of {len(times)} runs "
+                         f"({tc:.3f} s each)",
+               "seconds": round(tc, 4),
+               "gpu_vs_cpu": {"max_err_over_absAabsB": rel, "bound": 1e-3,
+                              "exact_zero_where_absAabsB_zero": exact_zero,
+                              "ok": rel <= 1e-3 and exact_zero}}
+
+    out = {
+        "metric": METRIC,
+        "value": round(tflops, 3),
+        "unit": "TFLOPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (reference generator, seeds 123/456)",
+        "config": {"workload": f"bcsr_spmm {M}x{K} {bh}x{bw} blocks {pattern} "
+                               f"{args.density * 100:g}% blocks N={N} {args.precision}",
+                   "M": M, "K": K, "N": N, "bh": bh, "bw": bw, "pattern": pattern,
+                   "block_density": args.density, "nnzb": int(h.nnzb),
+                   "tile_n": args.tile_n, "xcd_map": args.xcd_map, "parallelism": "single-gpu"},
+        "gbs": round(gbs, 1),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    stream.destroy()
+    B.free()
+    C.free()
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

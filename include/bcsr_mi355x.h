@@ -1,0 +1,202 @@
+/*
+ * bcsr_mi355x.h -- C ABI of the MI355X BCSR x dense engine.
+ *
+ * C = A_bcsr (M x K, bh x bw blocks) . B (K x N, fp16, row-major) -> fp32
+ * row-major M x N, every element of C written (empty block-rows give zeros).
+ *
+ * All functions return a bcsr_status (0 = ok, negative = error); none throws
+ * and none synchronises the stream it is given.  `stream` is a hipStream_t
+ * (NULL = the null stream).  Device pointers are HIP device addresses.
+ *
+ * Two ways in:
+ *  - plans: bcsr_plan_create* from host BCSR arrays, then bcsr_spmm*;
+ *  - drop-in: BCSRMatrix::copyToDevice (or bcsr_copy_to_device) returns
+ *    device hdr/idx/data arrays that the sparseMatrixMulTensor* entry points
+ *    below accept.  Those replace the reference's __global__ kernels of the
+ *    same names (src/cuda_kernels.cuh:39-114): same arguments plus a trailing
+ *    stream, and the library picks the launch geometry itself.
+ */
+#ifndef BCSR_MI355X_H
+#define BCSR_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bcsr_status {
+    BCSR_OK = 0,
+    BCSR_ERR_INVALID_ARG = -1,    /* null pointer, negative size, bad enum */
+    BCSR_ERR_BLOCK_SIZE = -2,     /* block size < 16, not a multiple of 16, or not the entry point's */
+    BCSR_ERR_SHAPE = -3,          /* M/N/K inconsistent with the plan, N % 16 != 0, misaligned B or C */
+    BCSR_ERR_HIP = -4,            /* a HIP runtime call failed (bcsr_last_hip_error) */
+    BCSR_ERR_OUT_OF_MEMORY = -5,  /* device or host allocation failed */
+    BCSR_ERR_NOT_REGISTERED = -6, /* device hdr pointer not produced by bcsr_copy_to_device */
+    BCSR_ERR_UNSUPPORTED = -7,    /* precision/variant not available for this plan */
+    BCSR_ERR_UNKNOWN_PATTERN = -8 /* generator pattern name not recognised */
+};
+
+/* Storage precision of A on the device.  B is fp16 and C fp32 in every mode. */
+enum bcsr_precision {
+    BCSR_PREC_F16 = 0,
+    BCSR_PREC_BF16 = 1,
+    BCSR_PREC_FP8_E4M3 = 2, /* per-block fp32 scale */
+    BCSR_PREC_F16_HILO = 3  /* hi + lo fp16 split, near-fp32 accuracy */
+};
+
+typedef struct bcsr_plan bcsr_plan;
+
+typedef struct {
+    int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 */
+    int32_t xcd_map;  /* -1 = auto, 0 = linear block order, 1 = column tiles pinned to XCDs */
+    int32_t variant;  /* 0 = default kernel; other values select alternative kernels */
+    int32_t reserved[5];
+} bcsr_launch_cfg;
+
+typedef struct {
+    int32_t rows, cols;             /* M, K */
+    int32_t bh, bw;                 /* block shape of the source BCSR */
+    int32_t block_rows, block_cols; /* rows / bh, cols / bw */
+    int32_t precision;              /* bcsr_precision */
+    int32_t dev_bh;                 /* rows per device block-row (16, 32 or 64) */
+    int64_t nnzb;                   /* stored bh x bw blocks */
+    int64_t dev_pairs;              /* device block pairs (K = 32 MFMA steps), padding included */
+    int64_t device_bytes;           /* device memory held by the plan */
+} bcsr_plan_info;
+
+const char *bcsr_status_string(int status);
+/* hipError_t of the last failing HIP call made by this library on this thread. */
+int bcsr_last_hip_error(void);
+
+/* ---- plans ---------------------------------------------------------------
+ * Host arrays in the reference layout: hdr[block_rows+1], idx[nnzb],
+ * data = nnzb row-major bh x bw blocks.  bcsr_plan_create takes fp16 data
+ * (uint16 bit patterns) and builds an fp16 plan; bcsr_plan_create_f32 takes
+ * fp32 block values and quantises them to `precision`. */
+int bcsr_plan_create(const int32_t *hdr, const int32_t *idx, const void *data_f16,
+                     int32_t block_rows, int32_t block_cols, int32_t bh, int32_t bw,
+                     bcsr_plan **out);
+int bcsr_plan_create_f32(const int32_t *hdr, const int32_t *idx, const float *data,
+                         int32_t block_rows, int32_t block_cols, int32_t bh, int32_t bw,
+                         int32_t precision, bcsr_plan **out);
+int bcsr_plan_destroy(bcsr_plan *plan);
+int bcsr_plan_get_info(const bcsr_plan *plan, bcsr_plan_info *info);
+
+/* C (device, M x N fp32) = plan . B (device, K x N fp16).  N % 16 == 0,
+ * B and C 16-byte aligned.  cfg may be NULL (automatic choice). */
+int bcsr_spmm(const bcsr_plan *plan, const void *B, float *C, int32_t N, void *stream);
+int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N,
+                  const bcsr_launch_cfg *cfg, void *stream);
+/* reps back-to-back launches of the same product on one stream (timing). */
+int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N,
+                     const bcsr_launch_cfg *cfg, int32_t reps, void *stream);
+/* Name of the kernel a launch with this configuration runs (for profiling). */
+int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
+                          char *buf, size_t buflen);
+
+/* ---- drop-in device arrays -------------------------------------------------
+ * Replaces BCSRMatrix::copyToDevice (formats/BCSRMatrix.cu:106-132): builds an
+ * fp16 plan from host arrays and returns its device arrays.  *gpu_hdr is the
+ * reference prefix array, *gpu_idx the block-column indices, *gpu_data the
+ * blocks repacked into MFMA operand order. */
+int bcsr_copy_to_device(const int32_t *hdr, const int32_t *idx, const void *data_f16,
+                        int32_t block_rows, int32_t block_cols, int32_t bh, int32_t bw,
+                        int **gpu_hdr, int **gpu_idx, void **gpu_data);
+/* Frees what bcsr_copy_to_device allocated. */
+int bcsr_release_device(const int *gpu_hdr);
+/* The plan behind a registered device hdr pointer (owned by the registry). */
+int bcsr_plan_from_device(const int *gpu_hdr, const bcsr_plan **out);
+
+/* ---- the reference's entry points ------------------------------------------
+ * hdr/idx/data must come from bcsr_copy_to_device (BCSR_ERR_NOT_REGISTERED
+ * otherwise); the block shape named by the entry point must match the
+ * registered matrix (BCSR_ERR_BLOCK_SIZE otherwise).  M is the row count of
+ * A and C, N the column count of B and C; the "1" variants take one n for
+ * square M = N = n.  colBlocksPerRow of the Adaptive kernels is accepted and
+ * ignored (the library balances work itself). */
+#define BCSR_ENTRY(name) \
+    int name(const int *hdr, const int *idx, const void *data, const void *B, float *C, \
+             unsigned int M, unsigned int N, void *stream)
+BCSR_ENTRY(sparseMatrixMulTensor);             /* src/cuda_kernels.cu:301 16x16 */
+BCSR_ENTRY(sparseMatrixMulTensor_v2);          /* src/cuda_kernels.cu:370 16x16 */
+BCSR_ENTRY(sparseMatrixMulTensor_v3);          /* src/cuda_kernels.cu:423 16x16 */
+BCSR_ENTRY(sparseMatrixMulTensor_v1_improved); /* src/cuda_kernels.cu:495 16x16 */
+BCSR_ENTRY(sparseMatrixMulTensor_option2_ldmatrix_sm80); /* src/cuda_kernels.cu:552 16x16 */
+BCSR_ENTRY(sparseMatrixMulTensor64x16);        /* src/cuda_kernels.cu:618 64x16 */
+BCSR_ENTRY(sparseMatrixMulTensor64x16_v2);     /* src/cuda_kernels.cu:939 64x16 */
+BCSR_ENTRY(sparseMatrixMulTensor32x16);        /* src/cuda_kernels.cu:699 32x16 */
+BCSR_ENTRY(sparseMatrixMulTensor32x16_shared); /* src/cuda_kernels.cu:768 32x16 */
+BCSR_ENTRY(sparseMatrixMulTensor32x16_shared_vectorized);    /* src/cuda_kernels.cu:853 */
+BCSR_ENTRY(sparseMatrixMulTensor32x16_v2);     /* src/cuda_kernels.cu:1001 32x16 */
+BCSR_ENTRY(sparseMatrixMulTensor32x16_v2_shared_vectorized); /* src/cuda_kernels.cu:1056 */
+#undef BCSR_ENTRY
+/* src/cuda_kernels.cu:330, square n x n */
+int sparseMatrixMulTensor1(const int *hdr, const int *idx, const void *data, const void *B,
+                           float *C, unsigned int n, void *stream);
+/* src/cuda_kernels.cu:1171, 1227, 1293 */
+int sparseMatrixMulTensorAdaptive(const int *hdr, const int *idx, const void *data, const void *B,
+                                  float *C, unsigned int M, unsigned int N,
+                                  unsigned int colBlocksPerRow, void *stream);
+int sparseMatrixMulTensorAdaptive32x16(const int *hdr, const int *idx, const void *data,
+                                       const void *B, float *C, unsigned int M, unsigned int N,
+                                       unsigned int colBlocksPerRow, void *stream);
+int sparseMatrixMulTensorAdaptive64x16(const int *hdr, const int *idx, const void *data,
+                                       const void *B, float *C, unsigned int M, unsigned int N,
+                                       unsigned int colBlocksPerRow, void *stream);
+
+/* ---- device plumbing ---------------------------------------------------------
+ * Thin wrappers over the HIP runtime this library is linked against, so that
+ * callers without their own HIP bindings (tests, bench, scripting languages)
+ * can allocate, copy, order and time work on the same runtime. */
+int bcsr_device_count(int32_t *count);
+int bcsr_set_device(int32_t device);
+int bcsr_dev_malloc(void **ptr, size_t bytes);
+int bcsr_dev_free(void *ptr);
+int bcsr_memcpy_htod(void *dst, const void *src, size_t bytes);
+int bcsr_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int bcsr_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int bcsr_memset_async(void *dst, int32_t byte_value, size_t bytes, void *stream);
+int bcsr_stream_create(void **stream);
+int bcsr_stream_destroy(void *stream);
+int bcsr_stream_synchronize(void *stream);
+int bcsr_device_synchronize(void);
+int bcsr_event_create(void **event);
+int bcsr_event_destroy(void *event);
+int bcsr_event_record(void *event, void *stream);
+int bcsr_event_elapsed_ms(void *start, void *end, float *ms);
+
+/* ---- host-side generator and formats ----------------------------------------
+ * mg::generate_matrix<T> (matrix_generator.cpp:28-140) written row-major as
+ * double; elem_type 0 float, 1 double, 2 int, 3 long long. */
+int mg_generate_matrix(int32_t rows, int32_t cols, double sparsity, const char *pattern,
+                       int32_t blocksize, uint32_t seed, int32_t elem_type, double *out);
+/* Extended float generator (mg::generate_into): bh x bw blocks, band
+ * half-width, in-block density, value_dist 0 uniform / 1 log-uniform on
+ * [lo, hi].  _f16 rounds double -> float -> half. */
+int mg_generate_f32(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,
+                    int32_t bw, int32_t band_halfwidth, double inblock_density,
+                    int32_t value_dist, double lo, double hi, uint32_t seed, float *out);
+int mg_generate_f16(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,
+                    int32_t bw, int32_t band_halfwidth, double inblock_density,
+                    int32_t value_dist, double lo, double hi, uint32_t seed, void *out);
+/* BCSRMatrix construction from dense fp16 (formats/BCSRMatrix.cu:22-74) in
+ * two calls: count (fills hdr[rows/bh + 1], returns nnzb) then fill. */
+int bcsr_host_count(const void *dense_f16, int32_t rows, int32_t cols, int32_t bh, int32_t bw,
+                    int32_t *hdr, int64_t *nnzb);
+int bcsr_host_fill(const void *dense_f16, int32_t rows, int32_t cols, int32_t bh, int32_t bw,
+                   const int32_t *hdr, int32_t *idx, void *data_f16);
+/* The ".mat" text format of Matrix(const string&) (Matrix.cu:14-32):
+ * "rows cols" then rows*cols values.  mat_read_header reads the shape,
+ * mat_read the fp16 values (and the count of values nonzero as floats),
+ * mat_write writes the format back.  BCSR_ERR_INVALID_ARG on a missing or
+ * malformed file. */
+int mat_read_header(const char *path, int32_t *rows, int32_t *cols);
+int mat_read(const char *path, int32_t rows, int32_t cols, void *out_f16, int32_t *nonzeros);
+int mat_write(const char *path, int32_t rows, int32_t cols, const void *data_f16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

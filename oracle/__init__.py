@@ -1,0 +1,237 @@
+"""ctypes loader of the oracle (liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+See oracle.h for what each function restates and what pins it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmg_ref.so")
+REFERENCE_TREE = "/root/reference"
+
+_o = None
+_r = None
+_P = ctypes.c_void_p
+
+
+def build(with_ref: bool | None = None):
+    """Compile liboracle.so (and oracle/_ref when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir(REFERENCE_TREE)
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REFERENCE_TREE}"], check=True)
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing; run `make -C oracle`")
+        o = ctypes.CDLL(LIB)
+        i, d, u32 = ctypes.c_int, ctypes.c_double, ctypes.c_uint32
+        sz = ctypes.c_size_t
+
+        def f(name, res, *args):
+            fn = getattr(o, name)
+            fn.restype = res
+            fn.argtypes = list(args)
+
+        f("orc_mt_stream", None, u32, _P, sz)
+        f("orc_f2h", ctypes.c_uint16, ctypes.c_float)
+        f("orc_h2f", ctypes.c_float, ctypes.c_uint16)
+        f("orc_f32_to_f16", None, _P, _P, sz)
+        f("orc_f16_to_f32", None, _P, _P, sz)
+        f("orc_generate_matrix", i, i, i, d, ctypes.c_char_p, i, u32, i, _P)
+        f("orc_generate_ex", i, i, i, d, ctypes.c_char_p, i, i, u32, i, d, d, i, d, _P)
+        f("orc_bcsr_count", ctypes.c_int64, _P, i, i, i, i, _P)
+        f("orc_bcsr_fill", None, _P, i, i, i, i, _P, _P, _P)
+        f("orc_bcsr_fill_f32", None, _P, i, i, i, i, _P, _P, _P)
+        f("orc_bcsr_densify", None, _P, _P, _P, i, i, i, i, _P)
+        f("orc_csr_count", ctypes.c_int64, _P, i, i, _P)
+        f("orc_csr_fill", None, _P, i, i, _P, _P, _P)
+        f("orc_matmul_dense_ref", None, _P, _P, i, i, i, _P)
+        f("orc_bcsr_spmm_f32", None, _P, _P, _P, i, i, i, i, _P, i, _P, i)
+        f("orc_bcsr_spmm_f64", None, _P, _P, _P, i, i, i, i, _P, i, _P, _P, i)
+        _o = o
+    return _o
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _r
+    if _r is None:
+        r = ctypes.CDLL(REF_LIB)
+        r.ref_generate_matrix.restype = ctypes.c_int
+        r.ref_generate_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_char_p, ctypes.c_int, ctypes.c_uint,
+                                          ctypes.c_int, _P]
+        r.ref_generate_matrix_f32.restype = ctypes.c_int
+        r.ref_generate_matrix_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_char_p, ctypes.c_int, ctypes.c_uint, _P]
+        _r = r
+    return _r
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+ELEM = {"float": 0, "double": 1, "int": 2, "longlong": 3}
+
+
+def mt_stream(seed, n):
+    out = np.empty(n, dtype=np.uint32)
+    lib().orc_mt_stream(seed, _p(out), n)
+    return out
+
+
+def generate_matrix(rows, cols, sparsity, pattern, blocksize, seed, elem_type="float"):
+    out = np.empty((rows, cols), dtype=np.float64)
+    rc = lib().orc_generate_matrix(rows, cols, sparsity, pattern.encode(), blocksize, seed,
+                                   ELEM[elem_type], _p(out))
+    if rc != 0:
+        raise ValueError(f"Unknown pattern: {pattern}")
+    return out
+
+
+def ref_generate_matrix(rows, cols, sparsity, pattern, blocksize, seed, elem_type="float"):
+    out = np.empty((rows, cols), dtype=np.float64)
+    rc = ref_lib().ref_generate_matrix(rows, cols, sparsity, pattern.encode(), blocksize, seed,
+                                       ELEM[elem_type], _p(out))
+    if rc != 0:
+        raise ValueError(f"Unknown pattern: {pattern}")
+    return out
+
+
+def ref_generate_f32(rows, cols, sparsity, pattern, blocksize, seed):
+    out = np.empty((rows, cols), dtype=np.float32)
+    rc = ref_lib().ref_generate_matrix_f32(rows, cols, sparsity, pattern.encode(), blocksize,
+                                           seed, _p(out))
+    if rc != 0:
+        raise ValueError(f"Unknown pattern: {pattern}")
+    return out
+
+
+def generate_ex(rows, cols, sparsity, pattern, bh=16, bw=None, seed=123, value_dist="uniform",
+                lo=1e-3, hi=1e2, band_halfwidth=8, inblock_density=1.0):
+    bw = bh if bw is None else bw
+    out = np.empty((rows, cols), dtype=np.float32)
+    vd = {"uniform": 0, "loguniform": 1}[value_dist]
+    rc = lib().orc_generate_ex(rows, cols, sparsity, pattern.encode(), bh, bw, seed, vd, lo, hi,
+                               band_halfwidth, inblock_density, _p(out))
+    if rc != 0:
+        raise ValueError(f"Unknown pattern: {pattern}")
+    return out
+
+
+def f32_to_f16_bits(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.empty(a.shape, dtype=np.uint16)
+    lib().orc_f32_to_f16(_p(a), _p(out), a.size)
+    return out
+
+
+def f16_bits_to_f32(h):
+    h = np.ascontiguousarray(h, dtype=np.uint16)
+    out = np.empty(h.shape, dtype=np.float32)
+    lib().orc_f16_to_f32(_p(h), _p(out), h.size)
+    return out
+
+
+def bcsr_from_dense(A16_bits, bh, bw):
+    """(hdr, idx, data uint16 [nnzb, bh, bw]) of BCSRMatrix(A, bh, bw)."""
+    A = np.ascontiguousarray(A16_bits, dtype=np.uint16)
+    rows, cols = A.shape
+    hdr = np.zeros(rows // bh + 1, dtype=np.int32)
+    n = lib().orc_bcsr_count(_p(A), rows, cols, bh, bw, _p(hdr))
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    data = np.zeros((max(n, 1), bh, bw), dtype=np.uint16)
+    lib().orc_bcsr_fill(_p(A), rows, cols, bh, bw, _p(hdr), _p(idx), _p(data))
+    return hdr, idx[:n], data[:n]
+
+
+def bcsr_values_f32(A32, hdr, idx, bh, bw):
+    A = np.ascontiguousarray(A32, dtype=np.float32)
+    rows, cols = A.shape
+    n = int(hdr[-1])
+    data = np.zeros((max(n, 1), bh, bw), dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if n else np.zeros(1, np.int32)
+    lib().orc_bcsr_fill_f32(_p(A), rows, cols, bh, bw, _p(np.ascontiguousarray(hdr, np.int32)),
+                            _p(idx), _p(data))
+    return data[:n]
+
+
+def bcsr_densify(hdr, idx, data, block_rows, block_cols, bh, bw):
+    out = np.empty((block_rows * bh, block_cols * bw), dtype=np.uint16)
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
+    data = np.ascontiguousarray(data, dtype=np.uint16) if len(data) else np.zeros(1, np.uint16)
+    lib().orc_bcsr_densify(_p(np.ascontiguousarray(hdr, np.int32)), _p(idx), _p(data), block_rows,
+                           block_cols, bh, bw, _p(out))
+    return out
+
+
+def csr_from_dense(A16_bits):
+    A = np.ascontiguousarray(A16_bits, dtype=np.uint16)
+    rows, cols = A.shape
+    hdr = np.zeros(rows + 1, dtype=np.int32)
+    n = lib().orc_csr_count(_p(A), rows, cols, _p(hdr))
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    data = np.zeros(max(n, 1), dtype=np.uint16)
+    lib().orc_csr_fill(_p(A), rows, cols, _p(hdr), _p(idx), _p(data))
+    return hdr, idx[:n], data[:n]
+
+
+def matmul_dense_ref(A16_bits, B16_bits):
+    """cpu_matmul_ref (nvbench_main.cu:78-91), literal dense loop."""
+    A = np.ascontiguousarray(A16_bits, dtype=np.uint16)
+    B = np.ascontiguousarray(B16_bits, dtype=np.uint16)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.empty((M, N), dtype=np.float32)
+    lib().orc_matmul_dense_ref(_p(A), _p(B), M, K, N, _p(C))
+    return C
+
+
+def bcsr_spmm_f32(hdr, idx, data16, bh, bw, Bf32, br0=0, br1=None, threads=1, C=None):
+    """BCSR-order fp32 product, bit-identical to cpu_matmul_ref.  Rows of
+    block-rows [br0, br1) of C are written."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.int32)
+    nbr = len(hdr) - 1
+    br1 = nbr if br1 is None else br1
+    Bf = np.ascontiguousarray(Bf32, dtype=np.float32)
+    N = Bf.shape[1]
+    if C is None:
+        C = np.zeros((nbr * bh, N), dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
+    data = np.ascontiguousarray(data16, dtype=np.uint16) if len(idx) else np.zeros(1, np.uint16)
+    lib().orc_bcsr_spmm_f32(_p(hdr), _p(idx), _p(data), bh, bw, br0, br1, _p(Bf), N, _p(C),
+                            threads)
+    return C
+
+
+def bcsr_spmm_f64(hdr, idx, data32, bh, bw, Bf32, br0=0, br1=None, threads=1, want_abs=True):
+    """FP64 product on fp32 block values and the |A|.|B| bound."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.int32)
+    nbr = len(hdr) - 1
+    br1 = nbr if br1 is None else br1
+    Bf = np.ascontiguousarray(Bf32, dtype=np.float32)
+    N = Bf.shape[1]
+    C = np.zeros((nbr * bh, N), dtype=np.float64)
+    Ca = np.zeros((nbr * bh, N), dtype=np.float64) if want_abs else None
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
+    data = np.ascontiguousarray(data32, dtype=np.float32) if len(idx) else np.zeros(1, np.float32)
+    lib().orc_bcsr_spmm_f64(_p(hdr), _p(idx), _p(data), bh, bw, br0, br1, _p(Bf), N, _p(C),
+                            _p(Ca) if want_abs else None, threads)
+    return C, Ca

@@ -1,0 +1,139 @@
+/*
+ * oracle.h -- CPU restatement of the reference's BCSR x dense path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * loaded only by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg.  The product (matrixmultiplicationcuda_amd) never calls it.
+ *
+ * What each function restates (reference file:line) and what pins it:
+ *
+ *   orc_mt_*              std::mt19937 as used by matrix_generator.cpp:34
+ *                         pinned: C++11 known answer (10000th output of the
+ *                         default-seeded engine is 4123659995) and numpy's
+ *                         independent MT19937 (RandomState legacy seeding).
+ *   orc_canonical         libstdc++ generate_canonical<double,53> (the engine
+ *                         adaptor behind uniform_real_distribution<double> and
+ *                         bernoulli_distribution, matrix_generator.cpp:20,50)
+ *                         pinned: oracle/_ref (the reference generator compiled
+ *                         from matrix_generator.cpp) on every pattern.
+ *   orc_uniform_int       libstdc++ uniform_int_distribution (Lemire downscale),
+ *                         matrix_generator.cpp:15,54.  pinned: oracle/_ref int.
+ *   orc_generate          mg::generate_matrix<T>, matrix_generator.cpp:28-140,
+ *                         all seven reference patterns; pinned: oracle/_ref
+ *                         bit-exact + committed golden hashes.
+ *                         Extensions with no reference counterpart (parity
+ *                         unpinned vs the reference; pinned by structural
+ *                         invariants and distribution checks in tests):
+ *                         "blockband", "blockrandom_rho", value dist "loguniform".
+ *   orc_f2h / orc_h2f     __float2half / __half2float (RNE), used by
+ *                         nvbench_main.cu:67-75 and Matrix.cu:30.
+ *                         pinned: numpy float16 conversion over all halves and
+ *                         a dense float sweep including ties and subnormals.
+ *   orc_bcsr_count/fill   BCSRMatrix::BCSRMatrix, formats/BCSRMatrix.cu:22-74 with
+ *                         blockDensity, miscutil.cpp:76-87.  pinned: invariants
+ *                         that fix the format uniquely + densify round trip.
+ *   orc_matmul_dense_ref  cpu_matmul_ref, nvbench_main.cu:78-91 (literal form).
+ *   orc_bcsr_spmm_f32     the same product in BCSR order; bit-identical to the
+ *                         dense form (fp16*fp16 is exact in fp32, skipped zero
+ *                         terms add +0, block columns ascend).  pinned: exact
+ *                         equality with orc_matmul_dense_ref, closed forms
+ *                         (identity blocks, one-hot B), numpy float64 matmul.
+ *   orc_bcsr_spmm_f64     FP64 oracle on the unquantised generator values.
+ *                         pinned: numpy float64 matmul.
+ *   orc_csr_*             CSRMatrix::CSRMatrix, formats/CSRMatrix.cu:13-40
+ *                         (column index taken modulo cols; the reference's
+ *                         "% rows" is only correct for square matrices).
+ */
+#ifndef BCSR_ORACLE_H
+#define BCSR_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint32_t mt[624];
+    int idx;
+} orc_mt;
+
+void orc_mt_seed(orc_mt *g, uint32_t seed);
+uint32_t orc_mt_next(orc_mt *g);
+/* Fill out[n] with consecutive engine outputs of mt19937(seed). */
+void orc_mt_stream(uint32_t seed, uint32_t *out, size_t n);
+double orc_canonical(orc_mt *g);
+int orc_bernoulli(orc_mt *g, double p);
+double orc_uniform(orc_mt *g, double a, double b);
+int64_t orc_uniform_int(orc_mt *g, int64_t a, int64_t b);
+
+uint16_t orc_f2h(float f);
+float orc_h2f(uint16_t h);
+void orc_f32_to_f16(const float *in, uint16_t *out, size_t n);
+void orc_f16_to_f32(const uint16_t *in, float *out, size_t n);
+
+/* Value distributions for generated nonzeros. */
+enum { ORC_VAL_UNIFORM = 0, ORC_VAL_LOGUNIFORM = 1 };
+/* Element types of the reference's generate_matrix<T> instantiations. */
+enum { ORC_T_FLOAT = 0, ORC_T_DOUBLE = 1, ORC_T_INT = 2, ORC_T_LONGLONG = 3 };
+
+/*
+ * mg::generate_matrix<T>(rows, cols, sparsity, pattern, blocksize, seed),
+ * written row-major as double into out[rows*cols] (a float/int result is
+ * exactly representable in double).  elem_type selects T.
+ * Returns 0, or -1 for an unknown pattern (the reference throws
+ * std::invalid_argument, matrix_generator.cpp:135-137).
+ */
+int orc_generate_matrix(int rows, int cols, double sparsity, const char *pattern,
+                        int blocksize, uint32_t seed, int elem_type, double *out);
+
+/*
+ * Extended generator (float only).  bh x bw block shape for the block
+ * patterns, band_w = half-width in block-columns for "blockband",
+ * rho = in-block element density for "blockrandom_rho", value_dist one of
+ * ORC_VAL_*, with [lo, hi] the magnitude range for ORC_VAL_LOGUNIFORM.
+ * Reference patterns reproduce orc_generate_matrix(..., ORC_T_FLOAT) when
+ * bh == bw == blocksize and value_dist == ORC_VAL_UNIFORM.
+ */
+int orc_generate_ex(int rows, int cols, double sparsity, const char *pattern,
+                    int bh, int bw, uint32_t seed, int value_dist, double lo,
+                    double hi, int band_w, double rho, float *out);
+
+/* BCSR construction from a dense row-major fp16 matrix. */
+int64_t orc_bcsr_count(const uint16_t *A, int rows, int cols, int bh, int bw,
+                       int32_t *hdr /* rows/bh + 1 */);
+void orc_bcsr_fill(const uint16_t *A, int rows, int cols, int bh, int bw,
+                   const int32_t *hdr, int32_t *idx, uint16_t *data);
+/* Same structure, float payload (for the FP64 oracle on unquantised values). */
+void orc_bcsr_fill_f32(const float *A, int rows, int cols, int bh, int bw,
+                       const int32_t *hdr, int32_t *idx, float *data);
+/* Densify: out[rows*cols] fp16 from BCSR. */
+void orc_bcsr_densify(const int32_t *hdr, const int32_t *idx, const uint16_t *data,
+                      int block_rows, int block_cols, int bh, int bw, uint16_t *out);
+
+/* CSR construction (column index = flat index % cols). */
+int64_t orc_csr_count(const uint16_t *A, int rows, int cols, int32_t *hdr);
+void orc_csr_fill(const uint16_t *A, int rows, int cols, const int32_t *hdr,
+                  int32_t *idx, uint16_t *data);
+
+/* cpu_matmul_ref (dense, literal loop order). */
+void orc_matmul_dense_ref(const uint16_t *A, const uint16_t *B, int M, int K, int N,
+                          float *C);
+/*
+ * BCSR-order fp32 product over block-rows [br0, br1) (C rows of those
+ * block-rows are overwritten), nthreads >= 1 splits the block-rows.
+ * Bf is B already widened to fp32 (K x N).
+ */
+void orc_bcsr_spmm_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data,
+                       int bh, int bw, int br0, int br1, const float *Bf, int N,
+                       float *C, int nthreads);
+/* FP64 product and |A|.|B| bound on float-valued BCSR data (same structure). */
+void orc_bcsr_spmm_f64(const int32_t *hdr, const int32_t *idx, const float *data,
+                       int bh, int bw, int br0, int br1, const float *Bf, int N,
+                       double *C, double *Cabs, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""GPU parity of the fp16 BCSR x dense kernels against the oracle.
+
+Every case goes through the C ABI (libbcsr_mi355x.so); the oracle's BCSR-order
+fp32 product is the reference CPU result (bit-identical to cpu_matmul_ref,
+nvbench_main.cu:78-91).  Bound per entry: 1e-3 * (|A|.|B|)_ij, exact zero
+where (|A|.|B|)_ij = 0, plus the reference's own |diff| <= 1e-2
+(nvbench_main.cu:413-420).
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import matrixmultiplicationcuda_amd as mm
+import oracle
+from helpers import assert_close, cpu_reference, make_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    assert mm.device_count() > 0, "GPU tests need a visible MI355X"
+
+
+def run_plan(case, cfg=None, precision="f16"):
+    _gpu()
+    plan = mm.Plan(case.h, precision)
+    B = mm.DeviceArray.from_numpy(case.B16)
+    C = mm.DeviceArray((case.M, case.N), np.float32)
+    C.fill_bytes(0xFF)  # NaN pattern: every entry must be written
+    plan.spmm(B, C, cfg=cfg)
+    mm.synchronize()
+    out = C.numpy()
+    plan.close()
+    B.free()
+    C.free()
+    return out
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_config0_against_dense_cpu_matmul_ref():
+    """config-0: 1024^2 blockrandom sp 0.9 16x16 seed 123, B 1024x256 seed 456."""
+    case = make_case(1024, 1024, 256, 0.9, "blockrandom", 16, 16)
+    assert case.h.nnzb == 400
+    log("config0 nnzb=400")
+    C = run_plan(case)
+    C_dense = oracle.matmul_dense_ref(case.A16.view(np.uint16), case.B16.view(np.uint16))
+    C_cpu, Cabs = cpu_reference(case)
+    assert np.array_equal(C_dense, C_cpu)  # the oracle's two forms agree bit for bit
+    rel = assert_close(C, C_cpu, Cabs, what="config0")
+    log(f"config0 max err / (|A||B|) = {rel:.3e}")
+
+
+@pytest.mark.parametrize("bh,pattern", [(16, "blockrandom"), (32, "pattern32by16"),
+                                        (64, "pattern64by16")])
+@pytest.mark.parametrize("N", [16, 48, 64, 128, 512])
+def test_block_shapes_and_widths(bh, pattern, N):
+    case = make_case(512, 768, N, 0.8, pattern, bh, 16)
+    log(f"shape bh={bh} N={N} nnzb={case.h.nnzb}")
+    C = run_plan(case)
+    C_cpu, Cabs = cpu_reference(case)
+    assert_close(C, C_cpu, Cabs, what=f"bh={bh} N={N}")
+
+
+@pytest.mark.parametrize("tile_n", [32, 64, 128, 256])
+@pytest.mark.parametrize("xcd_map", [0, 1])
+@pytest.mark.parametrize("bh", [16, 64])
+def test_launch_configs(tile_n, xcd_map, bh):
+    N = 512
+    case = make_case(1024, 1024, N, 0.85, "blockrandom" if bh == 16 else "pattern64by16", bh, 16)
+    log(f"cfg tile={tile_n} map={xcd_map} bh={bh}")
+    C = run_plan(case, cfg=mm.make_cfg(tile_n=tile_n, xcd_map=xcd_map))
+    C_cpu, Cabs = cpu_reference(case)
+    assert_close(C, C_cpu, Cabs, what=f"tile={tile_n} map={xcd_map}")
+
+
+@pytest.mark.parametrize("pattern", mm.REFERENCE_PATTERNS + mm.EXTRA_PATTERNS)
+def test_all_patterns(pattern):
+    case = make_case(640, 512, 96, 0.7, pattern, 16, 16, gen_bh=32 if pattern == "blockdiagonal" else 16,
+                     inblock_density=0.4)
+    log(f"pattern {pattern} nnzb={case.h.nnzb}")
+    C = run_plan(case)
+    C_cpu, Cabs = cpu_reference(case)
+    assert_close(C, C_cpu, Cabs, what=pattern)
+
+
+@pytest.mark.parametrize("bh,bw", [(32, 32), (48, 16), (128, 32), (16, 64)])
+def test_wide_and_tall_blocks(bh, bw):
+    """Blocks other than bh x 16 are cut into device blocks of 16 columns and
+    16/32/64 rows."""
+    case = make_case(384, 512, 64, 0.75, "blockrandom", bh, bw, gen_bh=16)
+    log(f"blocks {bh}x{bw} nnzb={case.h.nnzb}")
+    C = run_plan(case)
+    C_cpu, Cabs = cpu_reference(case)
+    assert_close(C, C_cpu, Cabs, what=f"{bh}x{bw}")
+
+
+def test_empty_matrix_writes_zeros():
+    case = make_case(256, 256, 64, 1.0, "blockrandom", 16, 16)
+    assert case.h.nnzb == 0
+    C = run_plan(case)
+    assert np.all(C == 0)
+
+
+def test_single_block_and_empty_rows():
+    A = np.zeros((128, 256), dtype=np.float16)
+    A[16:32, 48:64] = np.arange(256, dtype=np.float16).reshape(16, 16) / 256
+    case = make_case(128, 256, 32, 0.9)
+    case.A16 = A
+    case.h = mm.bcsr_from_dense(A, 16, 16)
+    assert case.h.nnzb == 1
+    C = run_plan(case)
+    C_cpu, Cabs = cpu_reference(case)
+    assert_close(C, C_cpu, Cabs, what="single block")
+    assert np.all(C[:16] == 0) and np.all(C[32:] == 0)
+
+
+def test_identity_blocks_select_rows_of_B():
+    """Closed form: A = identity made of 16x16 identity blocks -> C = B exactly."""
+    _gpu()
+    n = 512
+    A = np.eye(n, dtype=np.float16)
+    h = mm.bcsr_from_dense(A, 16, 16)
+    B = mm.generate(n, 128, 0.0, "random", seed=7)
+    plan = mm.Plan(h)
+    assert np.array_equal(plan.matmul(B), B.astype(np.float32))
+    plan.close()
+
+
+def test_drop_in_entry_points():
+    """bcsr_copy_to_device + every sparseMatrixMulTensor* name of the reference."""
+    _gpu()
+    lib = mm.lib()
+    for bh, pattern in [(16, "blockrandom"), (32, "pattern32by16"), (64, "pattern64by16")]:
+        case = make_case(512, 512, 128, 0.8, pattern, bh, 16)
+        h = case.h
+        ghdr, gidx, gdata = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        data = np.ascontiguousarray(h.data)
+        mm.check(lib.bcsr_copy_to_device(h.hdr.ctypes.data_as(ctypes.c_void_p),
+                                         h.idx.ctypes.data_as(ctypes.c_void_p),
+                                         data.ctypes.data_as(ctypes.c_void_p), h.block_rows,
+                                         h.block_cols, bh, 16, ctypes.byref(ghdr),
+                                         ctypes.byref(gidx), ctypes.byref(gdata)))
+        # the device hdr is the reference prefix array, idx the block-columns
+        hdr_back = np.empty_like(h.hdr)
+        idx_back = np.empty_like(h.idx)
+        mm.check(lib.bcsr_memcpy_dtoh(hdr_back.ctypes.data_as(ctypes.c_void_p), ghdr, hdr_back.nbytes))
+        mm.check(lib.bcsr_memcpy_dtoh(idx_back.ctypes.data_as(ctypes.c_void_p), gidx, idx_back.nbytes))
+        assert np.array_equal(hdr_back, h.hdr) and np.array_equal(idx_back, h.idx)
+        B = mm.DeviceArray.from_numpy(case.B16)
+        C_cpu, Cabs = cpu_reference(case)
+        stream = mm.Stream()
+        s = stream.handle
+        for name, ebh in list(mm.ENTRY_POINTS.items()) + list(mm.ADAPTIVE_ENTRY_POINTS.items()):
+            C = mm.DeviceArray((case.M, case.N), np.float32)
+            C.fill_bytes(0xFF)
+            mm.synchronize()
+            fn = getattr(lib, name)
+            args = [ghdr, gidx, gdata, B.ptr, C.ptr, case.M, case.N]
+            if name in mm.ADAPTIVE_ENTRY_POINTS:
+                args.append(2)
+            rc = fn(*args, s)
+            if ebh != bh:
+                assert rc == -2, f"{name} accepted a {bh}x16 matrix"
+                continue
+            assert rc == 0, f"{name}: {mm.STATUS.get(rc)}"
+            stream.synchronize()
+            assert_close(C.numpy(), C_cpu, Cabs, what=name)
+            C.free()
+        C = mm.DeviceArray((case.M, case.N), np.float32)
+        if bh == 16:
+            # sparseMatrixMulTensor1 is square n x n: M = N = n
+            rc = lib.sparseMatrixMulTensor1(ghdr, gidx, gdata, B.ptr, C.ptr, case.N, s)
+            assert rc == -3  # n = 128 != M = 512
+        # wrong M, unregistered hdr, misaligned B, N % 16 != 0
+        name = {16: "sparseMatrixMulTensor", 32: "sparseMatrixMulTensor32x16",
+                64: "sparseMatrixMulTensor64x16"}[bh]
+        fn = getattr(lib, name)
+        assert fn(ghdr, gidx, gdata, B.ptr, C.ptr, case.M + 16, case.N, s) == -3
+        assert fn(gidx, gidx, gdata, B.ptr, C.ptr, case.M, case.N, s) == -6
+        assert fn(ghdr, gidx, gdata, B.offset(2), C.ptr, case.M, case.N, s) == -3
+        assert fn(ghdr, gidx, gdata, B.ptr, C.ptr, case.M, case.N - 8, s) == -3
+        stream.synchronize()
+        stream.destroy()
+        C.free()
+        B.free()
+        mm.check(lib.bcsr_release_device(ghdr))
+        assert lib.bcsr_release_device(ghdr) == -6
+    mm.synchronize()
+
+
+def test_target_scale_sampled_rows_and_checksum():
+    """16384^2, 16x16, 5% blocks, N=512: sampled block-rows against the oracle
+    and the row-sum identity C.1 = A.(B.1) for every row."""
+    _gpu()
+    M = K = 16384
+    N = 512
+    case = make_case(M, K, N, 0.95, "blockrandom", 16, 16)
+    log(f"target nnzb={case.h.nnzb}")
+    plan = mm.Plan(case.h)
+    Cg = plan.matmul(case.B16)
+    rng = np.random.default_rng(5)
+    brs = sorted(set(rng.integers(0, M // 16, 48).tolist()) | {0, M // 16 - 1})
+    for br in brs:
+        C_cpu, Cabs = cpu_reference(case, br, br + 1)
+        sl = slice(br * 16, br * 16 + 16)
+        assert_close(Cg[sl], C_cpu[sl], Cabs[sl], what=f"block-row {br}")
+    # row sums: sum_j C[i, j] = sum_k A[i, k] * (sum_j B[k, j]) in fp64
+    bsum = case.B16.astype(np.float64).sum(axis=1)
+    babs = np.abs(case.B16.astype(np.float64)).sum(axis=1)
+    h = case.h
+    rows_ref = np.zeros(M)
+    rows_abs = np.zeros(M)
+    data = h.data.astype(np.float64)
+    for br in range(h.block_rows):
+        for k in range(h.hdr[br], h.hdr[br + 1]):
+            c = h.idx[k]
+            rows_ref[br * 16:(br + 1) * 16] += data[k] @ bsum[c * 16:(c + 1) * 16]
+            rows_abs[br * 16:(br + 1) * 16] += np.abs(data[k]) @ babs[c * 16:(c + 1) * 16]
+    got = Cg.astype(np.float64).sum(axis=1)
+    assert np.all(np.abs(got - rows_ref) <= 1e-3 * rows_abs + 1e-6)
+    plan.close()
