@@ -37,6 +37,10 @@ def parse():
     p.add_argument("--pattern", default=None,
                    help="generator pattern (default: blockrandom / pattern32by16 / pattern64by16 by bh)")
     p.add_argument("--tile-n", type=int, default=0)
+    p.add_argument("--variant", type=int, default=0, help="kernel variant (0 plain, 1 pipelined)")
+    p.add_argument("--calib-copy-mb", type=int, default=0,
+                   help="after timing, run the plain copy kernel over this many MiB once "
+                        "(reference for profiler byte counters)")
     p.add_argument("--xcd-map", type=int, default=-1)
     p.add_argument("--precision", default="f16", choices=["f16"])
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -75,7 +79,7 @@ def main():
 
     B = mm.DeviceArray.from_numpy(B16)
     C = mm.DeviceArray((M, N), np.float32)
-    cfg = mm.make_cfg(tile_n=args.tile_n, xcd_map=args.xcd_map)
+    cfg = mm.make_cfg(tile_n=args.tile_n, xcd_map=args.xcd_map, variant=args.variant)
     kname = plan.kernel_name(N, cfg)
     stream = mm.Stream()
     plan.spmm_repeat_ptr(B.ptr, C.ptr, N, args.warmup, stream, cfg)
@@ -91,6 +95,13 @@ def main():
     t = ms * 1e-3
     print(f"# timed {args.steps} steps: {ms * 1e3:.2f} us/step ({kname})", file=sys.stderr,
           flush=True)
+    if args.calib_copy_mb:
+        nb = args.calib_copy_mb << 20
+        src, dst = mm.DeviceArray((nb,), np.uint8), mm.DeviceArray((nb,), np.uint8)
+        mm.check(mm.lib().bcsr_copy_kernel(dst.ptr, src.ptr, nb, stream.handle), "copy kernel")
+        stream.synchronize()
+        src.free()
+        dst.free()
 
     flops = 2.0 * h.nnzb * bh * bw * N
     q = algorithmic_bytes(h.nnzb, bh, bw, M, K, N)
@@ -167,7 +178,8 @@ def main():
                                f"{args.density * 100:g}% blocks N={N} {args.precision}",
                    "M": M, "K": K, "N": N, "bh": bh, "bw": bw, "pattern": pattern,
                    "block_density": args.density, "nnzb": int(h.nnzb),
-                   "tile_n": args.tile_n, "xcd_map": args.xcd_map, "parallelism": "single-gpu"},
+                   "tile_n": args.tile_n, "xcd_map": args.xcd_map, "variant": args.variant,
+                   "parallelism": "single-gpu"},
         "gbs": round(gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -51,8 +51,9 @@ typedef struct bcsr_plan bcsr_plan;
 typedef struct {
     int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 */
     int32_t xcd_map;  /* -1 = auto, 0 = linear block order, 1 = column tiles pinned to XCDs */
-    int32_t variant;  /* 0 = default kernel; other values select alternative kernels */
-    int32_t reserved[5];
+    int32_t variant;  /* -1 = auto; 0 plain, 1 double-buffered, 2 64-bit addressing (16-bit modes) */
+    int32_t waves_per_block; /* 0 = auto; 1, 2 or 4 waves (independent work items) per workgroup */
+    int32_t reserved[4];
 } bcsr_launch_cfg;
 
 typedef struct {
@@ -92,6 +93,12 @@ int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N,
 /* reps back-to-back launches of the same product on one stream (timing). */
 int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N,
                      const bcsr_launch_cfg *cfg, int32_t reps, void *stream);
+/* The FP8 mode's B quantiser, on its own: B (device, K x N fp16) ->
+ * B8 (device, K * N bytes, 16-byte aligned), stored k-minor: byte
+ * (s * N + n) * 16 + r holds E4M3(B[16 s + r][n] / scale) -- and scales
+ * (device, (K / 16) * ceil(N / 256) bytes): the E8M0 exponent of each 16-row
+ * slab and 256-column chunk (smallest power of two with |B| / scale <= 448). */
+int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *stream);
 /* Name of the kernel a launch with this configuration runs (for profiling). */
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
                           char *buf, size_t buflen);
@@ -158,6 +165,10 @@ int bcsr_memcpy_htod(void *dst, const void *src, size_t bytes);
 int bcsr_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int bcsr_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int bcsr_memset_async(void *dst, int32_t byte_value, size_t bytes, void *stream);
+/* Device-to-device copy by a plain streaming kernel (bcsr_stream_copy);
+ * bytes and both pointers must be multiples of 16.  Also the known-size
+ * reference for hardware byte counters. */
+int bcsr_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 int bcsr_stream_create(void **stream);
 int bcsr_stream_destroy(void *stream);
 int bcsr_stream_synchronize(void *stream);

@@ -76,7 +76,8 @@ class BcsrError(RuntimeError):
 
 class bcsr_launch_cfg(ctypes.Structure):
     _fields_ = [("tile_n", ctypes.c_int32), ("xcd_map", ctypes.c_int32),
-                ("variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("variant", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class bcsr_plan_info(ctypes.Structure):
@@ -140,6 +141,8 @@ def _declare(lib):
     f("bcsr_memcpy_dtoh", i, _P, _P, ctypes.c_size_t)
     f("bcsr_memcpy_dtod", i, _P, _P, ctypes.c_size_t, _P)
     f("bcsr_memset_async", i, _P, _I32, ctypes.c_size_t, _P)
+    f("bcsr_copy_kernel", i, _P, _P, ctypes.c_size_t, _P)
+    f("bcsr_quantize_b_e4m3", i, _P, _I32, _I32, _P, _P, _P)
     f("bcsr_stream_create", i, ctypes.POINTER(_P))
     f("bcsr_stream_destroy", i, _P)
     f("bcsr_stream_synchronize", i, _P)
@@ -369,9 +372,11 @@ def stream_handle(stream):
     return ctypes.c_void_p(int(stream))
 
 
-def make_cfg(tile_n=0, xcd_map=-1, variant=0):
+def make_cfg(tile_n=0, xcd_map=-1, variant=-1, waves_per_block=0):
+    """Launch configuration; 0 / -1 fields are chosen automatically."""
     cfg = bcsr_launch_cfg()
     cfg.tile_n, cfg.xcd_map, cfg.variant = tile_n, xcd_map, variant
+    cfg.waves_per_block = waves_per_block
     return cfg
 
 

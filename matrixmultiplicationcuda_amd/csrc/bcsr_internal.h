@@ -1,0 +1,128 @@
+// bcsr_internal.h -- types shared by the plan builder and the kernel files.
+//
+// Device layout of a plan
+//   A bh x bw BCSR matrix is viewed as dev_bh x 16 device blocks (dev_bh = 64,
+//   32 or 16, the largest that divides bh; bw is cut into 16-column pieces).
+//   A device block-row's blocks (ascending block-column) are taken G at a time
+//   (G = 2 "pairs" for the 16-bit modes: one K = 32 step of the 16x16x32 MFMA;
+//   G = 4 "quads" for FP8: one K = 128 step of the scaled 16x16x128 MFMA with
+//   one block per 32-wide K group); short groups are padded with zero blocks
+//   that repeat the last block-column.
+//     grp_ptr[dev_rows + 1]  prefix count of groups per device block-row
+//     grp_col[G * groups]    16-column block index of every group slot
+//     vals                   the A operands, one 16-byte chunk per lane:
+//                            pairs: [group][row-subtile s][lane l][8 x 16-bit]
+//                              = rows 16 s + l % 16, columns 8 (l / 16) .. + 7
+//                              of the pair's 16 x 32 slice (hi+lo mode stores
+//                              [group][s][hi, lo][lane]);
+//                            quads: [group][s][lane l][16 x fp8] = row
+//                              16 s + l % 16 of slot l / 16's block;
+//     ascale                 FP8 only: per group the four slots' E8M0 block
+//                            scales, byte g = slot g;
+//     row_order[dev_rows]    device block-rows by descending group count.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+
+#include "bcsr_mi355x.h"
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+struct bcsr_plan {
+    bcsr_plan_info info{};
+    int device = 0;
+    int rs = 1;        // dev_bh / 16
+    int dev_rows = 0;  // M / dev_bh
+    int group = 2;     // blocks per group: 2 (pairs) or 4 (FP8 quads)
+    void *dev_base = nullptr;
+    int32_t *d_hdr = nullptr;  // reference prefix array (API view)
+    int32_t *d_idx = nullptr;  // reference block-column indices (API view)
+    int32_t *d_grp_ptr = nullptr;
+    int32_t *d_grp_col = nullptr;
+    int32_t *d_row_order = nullptr;
+    void *d_vals = nullptr;
+    uint32_t *d_ascale = nullptr;
+    // FP8: workspace holding the quantised B of the current call
+    std::mutex *ws_mu = nullptr;
+    void *d_ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+// Launch geometry resolved from a bcsr_launch_cfg.
+struct LaunchShape {
+    int variant = 0;
+    int wpb = 1;  // waves (work items) per workgroup
+    int epl = 8;  // output columns per lane; tile width = 16 * epl
+    int map_mode = 0;
+    int n_tiles = 1;
+    int grid = 0;
+};
+
+struct SpmmArgs {
+    const int32_t *grp_ptr;
+    const int32_t *grp_col;
+    const u32x4 *vals;
+    const uint32_t *ascale;
+    const int32_t *row_order;
+    const _Float16 *B;       // fp16 B (pair kernels)
+    const uint8_t *B8;       // quantised B, k-minor panels (FP8 kernel)
+    const uint8_t *b8scale;  // E8M0 per (16-row slab, 256-column chunk)
+    float *C;
+    int N;
+    int n_tiles;
+    int dev_rows;
+    int map_mode;
+    int b_bytes;  // bytes of B (or B8) when below 2^31: buffer-load bound
+    int b_chunks;  // 256-column chunks of B8 scales
+    int wpb;       // waves per workgroup
+};
+
+int hip_status(hipError_t e);
+int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls);
+int launch_pairs(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N,
+                 hipStream_t stream);
+int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream);
+bool pair_shape_supported(int precision, int rs, int epl, int variant);
+void pair_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
+void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
+
+// Work item of the calling wave: (column tile, slot in row_order).  Wave w of
+// workgroup b is item (b >> 3) * wpb + w of XCD b & 7 (workgroups are
+// dispatched round-robin over the 8 XCDs).  map_mode 0 ignores the XCD and
+// numbers items b * wpb + w, tile fastest; map_mode 1 pins every column tile
+// to XCDs: n_tiles % 8 == 0 gives XCD x the tiles x, x + 8, ...; n_tiles in
+// {1, 2, 4} lets 8 / n_tiles XCDs share each tile.
+__device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
+    const int bid = blockIdx.x, w = threadIdx.x >> 6;
+    if (a.map_mode == 0) {
+        const int item = bid * a.wpb + w;
+        tile = item % a.n_tiles;
+        slot = item / a.n_tiles;
+    } else {
+        const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
+        if (a.n_tiles >= 8) {
+            const int per = a.n_tiles >> 3;
+            tile = x + 8 * (i % per);
+            slot = i / per;
+        } else {
+            const int q = 8 / a.n_tiles;
+            tile = x % a.n_tiles;
+            slot = i * q + x / a.n_tiles;
+        }
+    }
+    return slot < a.dev_rows;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
+    // gfx9 buffer descriptor word 3; num_records = bytes (reads past it return 0)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), static_cast<short>(0), bytes, 0x00020000);
+}

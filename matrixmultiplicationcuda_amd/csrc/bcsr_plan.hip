@@ -1,0 +1,598 @@
+// bcsr_plan.hip -- device layout of a BCSR matrix (the plan), the registry
+// behind the reference-named entry points, and the C ABI.
+//
+// Replaces BCSRMatrix::copyToDevice (formats/BCSRMatrix.cu:106-132) and the
+// launch sites of the sparseMatrixMulTensor* kernels (src/cuda_kernels.cuh:
+// 39-114): the caller no longer picks grids, the plan does.  The layout is
+// described in bcsr_internal.h; the kernels live in kernels_pair.hip (16-bit
+// modes) and kernels_fp8.hip (FP8 mode).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "BCSRMatrix.h"
+#include "CSRMatrix.h"
+#include "bcsr_internal.h"
+
+namespace {
+thread_local int g_last_hip_error = 0;
+}
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return BCSR_OK;
+    g_last_hip_error = static_cast<int>(e);
+    return e == hipErrorOutOfMemory ? BCSR_ERR_OUT_OF_MEMORY : BCSR_ERR_HIP;
+}
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+uint16_t f32_to_f16_bits(float f) {
+    _Float16 h = static_cast<_Float16>(f);
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+
+float f16_bits_to_f32(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return static_cast<float>(h);
+}
+
+// float -> bfloat16, round to nearest even (NaN kept quiet).
+uint16_t f32_to_bf16_bits(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return static_cast<uint16_t>(x >> 16);
+}
+
+}  // namespace
+
+// float -> FP8 E4M3 (OCP: bias 7, max 448, NaN = 0x7f, no infinity), round
+// to nearest even; finite overflow saturates to +-448.
+uint8_t bcsr_f32_to_e4m3(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint8_t sign = static_cast<uint8_t>((x >> 24) & 0x80u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return sign | 0x7fu;
+    if (ax >= 0x43e00000u) return sign | 0x7eu;
+    if (ax < 0x3c800000u) {  // below the smallest normal 2^-6: multiples of 2^-9
+        const uint32_t e = ax >> 23;
+        if (e < 116u) return sign;
+        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 141u - e;
+        uint32_t r = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (r & 1u))) r++;
+        return sign | static_cast<uint8_t>(r);
+    }
+    const uint32_t e = ax >> 23;
+    uint32_t r = ((e - 120u) << 3) | ((ax >> 20) & 7u);
+    const uint32_t rem = ax & 0xfffffu;
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) r++;
+    return sign | static_cast<uint8_t>(std::min<uint32_t>(r, 0x7eu));
+}
+
+// E8M0 exponent of the smallest power of two s with absmax / s <= 448.
+uint8_t bcsr_e8m0_for_absmax(float absmax) {
+    if (!(absmax > 0.0f)) return 127;
+    int k = -127;
+    while (k < 127 && std::ldexp(448.0f, k) < absmax) k++;
+    return static_cast<uint8_t>(k + 127);
+}
+
+namespace {
+
+// Source block values, fp16 bits or fp32.
+struct SrcBlocks {
+    const uint16_t *f16 = nullptr;
+    const float *f32 = nullptr;
+    float value(int64_t i) const { return f16 ? f16_bits_to_f32(f16[i]) : f32[i]; }
+    uint16_t half_bits(int64_t i) const { return f16 ? f16[i] : f32_to_f16_bits(f32[i]); }
+};
+
+int validate_host(const int32_t *hdr, const int32_t *idx, int32_t block_rows, int32_t block_cols, int32_t bh,
+                  int32_t bw) {
+    if (!hdr || block_rows < 0 || block_cols < 0) return BCSR_ERR_INVALID_ARG;
+    if (bh < 16 || bw < 16 || bh % 16 || bw % 16) return BCSR_ERR_BLOCK_SIZE;
+    if (static_cast<int64_t>(block_rows) * bh > INT32_MAX || static_cast<int64_t>(block_cols) * bw > INT32_MAX)
+        return BCSR_ERR_SHAPE;
+    if (hdr[0] != 0) return BCSR_ERR_INVALID_ARG;
+    for (int i = 0; i < block_rows; ++i) {
+        if (hdr[i + 1] < hdr[i]) return BCSR_ERR_INVALID_ARG;
+        for (int64_t k = hdr[i]; k < hdr[i + 1]; ++k) {
+            if (!idx || idx[k] < 0 || idx[k] >= block_cols) return BCSR_ERR_INVALID_ARG;
+            if (k > hdr[i] && idx[k] <= idx[k - 1]) return BCSR_ERR_INVALID_ARG;
+        }
+    }
+    return BCSR_OK;
+}
+
+int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int32_t block_rows,
+               int32_t block_cols, int32_t bh, int32_t bw, int32_t precision, bcsr_plan **out) {
+    if (!out) return BCSR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int rc = validate_host(hdr, idx, block_rows, block_cols, bh, bw);
+    if (rc != BCSR_OK) return rc;
+    if (precision < BCSR_PREC_F16 || precision > BCSR_PREC_F16_HILO) return BCSR_ERR_INVALID_ARG;
+
+    const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
+    const int G = fp8 ? 4 : 2;                                  // blocks per group
+    const int H = precision == BCSR_PREC_F16_HILO ? 2 : 1;      // operand copies (hi, lo)
+    const int dev_bh = bh % 64 == 0 ? 64 : (bh % 32 == 0 ? 32 : 16);
+    const int rs = dev_bh / 16;
+    const int sub_rows = bh / dev_bh;
+    const int pieces = bw / 16;
+    const int dev_rows = block_rows * sub_rows;
+    const int64_t nnzb = hdr[block_rows];
+    const int64_t blk_elems = static_cast<int64_t>(bh) * bw;
+
+    // FP8: one E8M0 scale per source block
+    std::vector<uint8_t> blk_scale;
+    if (fp8) {
+        blk_scale.resize(nnzb);
+        for (int64_t k = 0; k < nnzb; ++k) {
+            float amax = 0.0f;
+            for (int64_t i = 0; i < blk_elems; ++i) amax = std::max(amax, std::fabs(src.value(k * blk_elems + i)));
+            blk_scale[k] = bcsr_e8m0_for_absmax(amax);
+        }
+    }
+
+    std::vector<int32_t> grp_ptr(dev_rows + 1, 0);
+    for (int R = 0; R < block_rows; ++R) {
+        const int64_t nblk = static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces;
+        for (int t = 0; t < sub_rows; ++t) grp_ptr[R * sub_rows + t + 1] = static_cast<int32_t>((nblk + G - 1) / G);
+    }
+    for (int r = 0; r < dev_rows; ++r) grp_ptr[r + 1] += grp_ptr[r];
+    const int64_t groups = grp_ptr[dev_rows];
+    const size_t chunk_bytes = 16;                                 // per lane per operand
+    const size_t group_bytes = static_cast<size_t>(rs) * H * 64 * chunk_bytes;
+
+    std::vector<int32_t> grp_col(static_cast<size_t>(G) * groups);
+    std::vector<uint8_t> vals(static_cast<size_t>(groups) * group_bytes, 0);
+    std::vector<uint32_t> ascale(fp8 ? groups : 0, 0);
+    for (int R = 0; R < block_rows; ++R) {
+        const int64_t k0 = hdr[R], nblk = static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces;
+        for (int t = 0; t < sub_rows; ++t) {
+            int64_t p = grp_ptr[R * sub_rows + t];
+            for (int64_t b = 0; b < nblk; b += G, ++p) {
+                uint8_t *gv = vals.data() + static_cast<size_t>(p) * group_bytes;
+                for (int slot = 0; slot < G; ++slot) {
+                    int64_t db = b + slot;
+                    const bool pad = db >= nblk;
+                    if (pad) db = nblk - 1;
+                    const int64_t k = k0 + db / pieces;
+                    const int q = static_cast<int>(db % pieces);
+                    grp_col[static_cast<size_t>(G) * p + slot] = idx[k] * pieces + q;
+                    if (fp8) ascale[p] |= static_cast<uint32_t>(pad ? 127 : blk_scale[k]) << (8 * slot);
+                    if (pad) continue;
+                    const float inv = fp8 ? std::ldexp(1.0f, 127 - static_cast<int>(blk_scale[k])) : 1.0f;
+                    for (int s = 0; s < rs; ++s)
+                        for (int lane = 0; lane < 64; ++lane) {
+                            const int g = lane >> 4, i = lane & 15;
+                            const int row = t * dev_bh + 16 * s + i;
+                            if (fp8) {
+                                if (g != slot) continue;
+                                uint8_t *dst = gv + (static_cast<size_t>(s) * 64 + lane) * chunk_bytes;
+                                for (int e = 0; e < 16; ++e) {
+                                    const int64_t si = k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + e;
+                                    dst[e] = bcsr_f32_to_e4m3(src.value(si) * inv);
+                                }
+                                continue;
+                            }
+                            if ((g >> 1) != slot) continue;
+                            for (int h = 0; h < H; ++h) {
+                                uint16_t *dst = reinterpret_cast<uint16_t *>(
+                                    gv + ((static_cast<size_t>(s) * H + h) * 64 + lane) * chunk_bytes);
+                                for (int e = 0; e < 8; ++e) {
+                                    const int64_t si =
+                                        k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + (g & 1) * 8 + e;
+                                    if (precision == BCSR_PREC_F16) {
+                                        dst[e] = src.half_bits(si);
+                                    } else if (precision == BCSR_PREC_BF16) {
+                                        dst[e] = f32_to_bf16_bits(src.value(si));
+                                    } else {  // hi + lo: a = hi + 2^-11 lo
+                                        const float a = src.value(si);
+                                        const uint16_t hi = f32_to_f16_bits(a);
+                                        dst[e] = h == 0 ? hi : f32_to_f16_bits((a - f16_bits_to_f32(hi)) * 2048.0f);
+                                    }
+                                }
+                            }
+                        }
+                }
+            }
+        }
+    }
+
+    std::vector<int32_t> row_order(dev_rows);
+    for (int r = 0; r < dev_rows; ++r) row_order[r] = r;
+    std::stable_sort(row_order.begin(), row_order.end(),
+                     [&](int x, int y) { return grp_ptr[x + 1] - grp_ptr[x] > grp_ptr[y + 1] - grp_ptr[y]; });
+
+    bcsr_plan *plan = new (std::nothrow) bcsr_plan();
+    if (!plan) return BCSR_ERR_OUT_OF_MEMORY;
+    const size_t A = 256;
+    const size_t off_hdr = 0;
+    const size_t off_idx = align_up(off_hdr + sizeof(int32_t) * (block_rows + 1), A);
+    const size_t off_gp = align_up(off_idx + sizeof(int32_t) * std::max<int64_t>(nnzb, 1), A);
+    const size_t off_gc = align_up(off_gp + sizeof(int32_t) * (dev_rows + 1), A);
+    const size_t off_ro = align_up(off_gc + sizeof(int32_t) * std::max<size_t>(grp_col.size(), 4), A);
+    const size_t off_as = align_up(off_ro + sizeof(int32_t) * std::max(dev_rows, 1), A);
+    const size_t off_v = align_up(off_as + sizeof(uint32_t) * std::max<size_t>(ascale.size(), 1), A);
+    const size_t total = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
+
+    int dev = 0;
+    rc = hip_status(hipGetDevice(&dev));
+    if (rc == BCSR_OK) rc = hip_status(hipMalloc(&plan->dev_base, total));
+    if (rc != BCSR_OK) {
+        delete plan;
+        return rc;
+    }
+    char *base = static_cast<char *>(plan->dev_base);
+    plan->device = dev;
+    plan->d_hdr = reinterpret_cast<int32_t *>(base + off_hdr);
+    plan->d_idx = reinterpret_cast<int32_t *>(base + off_idx);
+    plan->d_grp_ptr = reinterpret_cast<int32_t *>(base + off_gp);
+    plan->d_grp_col = reinterpret_cast<int32_t *>(base + off_gc);
+    plan->d_row_order = reinterpret_cast<int32_t *>(base + off_ro);
+    plan->d_ascale = fp8 ? reinterpret_cast<uint32_t *>(base + off_as) : nullptr;
+    plan->d_vals = base + off_v;
+    struct Copy {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } copies[] = {{plan->d_hdr, hdr, sizeof(int32_t) * (block_rows + 1)},
+                  {plan->d_idx, idx, sizeof(int32_t) * static_cast<size_t>(nnzb)},
+                  {plan->d_grp_ptr, grp_ptr.data(), sizeof(int32_t) * (dev_rows + 1)},
+                  {plan->d_grp_col, grp_col.data(), sizeof(int32_t) * grp_col.size()},
+                  {plan->d_row_order, row_order.data(), sizeof(int32_t) * dev_rows},
+                  {base + off_as, ascale.data(), sizeof(uint32_t) * ascale.size()},
+                  {plan->d_vals, vals.data(), vals.size()}};
+    for (const Copy &c : copies) {
+        if (!c.bytes) continue;
+        rc = hip_status(hipMemcpy(c.dst, c.src, c.bytes, hipMemcpyHostToDevice));
+        if (rc != BCSR_OK) {
+            (void)hipFree(plan->dev_base);
+            delete plan;
+            return rc;
+        }
+    }
+    plan->rs = rs;
+    plan->dev_rows = dev_rows;
+    plan->group = G;
+    if (fp8) plan->ws_mu = new std::mutex();
+    bcsr_plan_info &inf = plan->info;
+    inf.rows = block_rows * bh;
+    inf.cols = block_cols * bw;
+    inf.bh = bh;
+    inf.bw = bw;
+    inf.block_rows = block_rows;
+    inf.block_cols = block_cols;
+    inf.precision = precision;
+    inf.dev_bh = dev_bh;
+    inf.nnzb = nnzb;
+    inf.dev_pairs = groups;
+    inf.device_bytes = static_cast<int64_t>(total);
+    *out = plan;
+    return BCSR_OK;
+}
+
+int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N) {
+    if (!plan) return BCSR_ERR_INVALID_ARG;
+    if (N < 0) return BCSR_ERR_INVALID_ARG;
+    if (N % 16 != 0) return BCSR_ERR_SHAPE;
+    if (N == 0 || plan->info.rows == 0) return BCSR_OK;
+    if (!B || !C) return BCSR_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(B) | reinterpret_cast<uintptr_t>(C)) & 15u) return BCSR_ERR_SHAPE;
+    return BCSR_OK;
+}
+
+int launch_spmm(const bcsr_plan *plan, const void *B, float *C, int N, const bcsr_launch_cfg *cfg,
+                hipStream_t stream) {
+    int rc = check_operands(plan, B, C, N);
+    if (rc != BCSR_OK) return rc;
+    if (N == 0 || plan->info.rows == 0) return BCSR_OK;
+    LaunchShape ls;
+    rc = resolve_launch(plan, N, cfg, ls);
+    if (rc != BCSR_OK) return rc;
+    if (plan->info.precision == BCSR_PREC_FP8_E4M3)
+        return launch_fp8(const_cast<bcsr_plan *>(plan), ls, B, C, N, stream);
+    return launch_pairs(plan, ls, B, C, N, stream);
+}
+
+// Registry of plans made by bcsr_copy_to_device, keyed by their device hdr.
+std::mutex g_reg_mu;
+std::unordered_map<const int *, bcsr_plan *> &registry() {
+    static std::unordered_map<const int *, bcsr_plan *> r;
+    return r;
+}
+
+// bh > 0 additionally requires a bh x 16 matrix (the reference's kernels
+// hard-code 16-column blocks).
+int lookup(const int *hdr, int bh, bcsr_plan **plan) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto it = registry().find(hdr);
+    if (it == registry().end()) return BCSR_ERR_NOT_REGISTERED;
+    if (bh > 0 && (it->second->info.bh != bh || it->second->info.bw != 16)) return BCSR_ERR_BLOCK_SIZE;
+    *plan = it->second;
+    return BCSR_OK;
+}
+
+int entry(int bh, const int *hdr, const int *idx, const void *data, const void *B, float *C, unsigned M,
+          unsigned N, void *stream) {
+    bcsr_plan *plan = nullptr;
+    int rc = lookup(hdr, bh, &plan);
+    if (rc != BCSR_OK) return rc;
+    if (idx != plan->d_idx || data != plan->d_vals) return BCSR_ERR_INVALID_ARG;
+    if (M != static_cast<unsigned>(plan->info.rows) || N > static_cast<unsigned>(INT32_MAX)) return BCSR_ERR_SHAPE;
+    return launch_spmm(plan, B, C, static_cast<int>(N), nullptr, static_cast<hipStream_t>(stream));
+}
+
+}  // namespace
+
+int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls) {
+    int tile_n = cfg ? cfg->tile_n : 0;
+    if (tile_n == 0) tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
+    if (cfg && cfg->tile_n == 0 && plan->rs == 4 && tile_n == 128 && plan->info.precision == BCSR_PREC_F16_HILO)
+        tile_n = 64;
+    if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
+    // automatic choice: the prefetching kernel for 16-row device blocks, the
+    // plain one for 32/64-row blocks (whose larger register tiles leave no room
+    // for a second set of loads)
+    const bool fp8 = plan->info.precision == BCSR_PREC_FP8_E4M3;
+    const bool auto_variant = !cfg || cfg->variant < 0;
+    ls.variant = auto_variant ? (plan->rs == 1 ? 1 : 0) : cfg->variant;
+    if (ls.variant > 2) return BCSR_ERR_UNSUPPORTED;
+    if (!fp8 && !pair_shape_supported(plan->info.precision, plan->rs, tile_n / 16, ls.variant)) {
+        if (!auto_variant || ls.variant != 1) return BCSR_ERR_UNSUPPORTED;
+        ls.variant = 0;
+        if (!pair_shape_supported(plan->info.precision, plan->rs, tile_n / 16, 0)) return BCSR_ERR_UNSUPPORTED;
+    }
+    if (fp8 && ls.variant == 2) return BCSR_ERR_UNSUPPORTED;
+    // buffer offsets are 32-bit: larger B falls back to 64-bit global loads
+    if (!fp8 && static_cast<int64_t>(plan->info.cols) * N * 2 > INT32_MAX) ls.variant = 2;
+    if (fp8 && static_cast<int64_t>(plan->info.cols) * N > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
+    ls.epl = tile_n / 16;
+    ls.n_tiles = (N + tile_n - 1) / tile_n;
+    const bool xcd_ok = ls.n_tiles == 1 || ls.n_tiles == 2 || ls.n_tiles == 4 || ls.n_tiles % 8 == 0;
+    int map = cfg ? cfg->xcd_map : -1;
+    if (map < 0) map = xcd_ok ? 1 : 0;
+    if (map == 1 && !xcd_ok) return BCSR_ERR_INVALID_ARG;
+    if (map != 0 && map != 1) return BCSR_ERR_INVALID_ARG;
+    ls.map_mode = map;
+    const int wpb = cfg && cfg->waves_per_block > 0 ? cfg->waves_per_block : 1;
+    if (wpb != 1 && wpb != 2 && wpb != 4) return BCSR_ERR_INVALID_ARG;
+    ls.wpb = wpb;
+    const int rows = plan->dev_rows;
+    if (map == 0) {
+        ls.grid = (ls.n_tiles * rows + wpb - 1) / wpb;
+    } else if (ls.n_tiles >= 8) {
+        const int per_xcd = ls.n_tiles / 8 * rows;  // items of one XCD
+        ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
+    } else {
+        const int q = 8 / ls.n_tiles;
+        const int per_xcd = (rows + q - 1) / q;
+        ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
+    }
+    return BCSR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *bcsr_status_string(int status) {
+    switch (status) {
+        case BCSR_OK: return "ok";
+        case BCSR_ERR_INVALID_ARG: return "invalid argument";
+        case BCSR_ERR_BLOCK_SIZE: return "unsupported block size";
+        case BCSR_ERR_SHAPE: return "inconsistent shape or misaligned operand";
+        case BCSR_ERR_HIP: return "HIP runtime error";
+        case BCSR_ERR_OUT_OF_MEMORY: return "out of memory";
+        case BCSR_ERR_NOT_REGISTERED: return "device hdr pointer not registered by copyToDevice";
+        case BCSR_ERR_UNSUPPORTED: return "unsupported precision or variant";
+        case BCSR_ERR_UNKNOWN_PATTERN: return "unknown pattern";
+        default: return "unknown status";
+    }
+}
+
+int bcsr_last_hip_error(void) { return g_last_hip_error; }
+
+int bcsr_plan_create(const int32_t *hdr, const int32_t *idx, const void *data_f16, int32_t block_rows,
+                     int32_t block_cols, int32_t bh, int32_t bw, bcsr_plan **out) {
+    if (!data_f16 && hdr && block_rows > 0 && hdr[block_rows] > 0) return BCSR_ERR_INVALID_ARG;
+    SrcBlocks src;
+    src.f16 = static_cast<const uint16_t *>(data_f16);
+    try {
+        return build_plan(hdr, idx, src, block_rows, block_cols, bh, bw, BCSR_PREC_F16, out);
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
+}
+
+int bcsr_plan_create_f32(const int32_t *hdr, const int32_t *idx, const float *data, int32_t block_rows,
+                         int32_t block_cols, int32_t bh, int32_t bw, int32_t precision, bcsr_plan **out) {
+    if (!data && hdr && block_rows > 0 && hdr[block_rows] > 0) return BCSR_ERR_INVALID_ARG;
+    if (precision < BCSR_PREC_F16 || precision > BCSR_PREC_F16_HILO) return BCSR_ERR_INVALID_ARG;
+    SrcBlocks src;
+    src.f32 = data;
+    try {
+        return build_plan(hdr, idx, src, block_rows, block_cols, bh, bw, precision, out);
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
+}
+
+int bcsr_plan_destroy(bcsr_plan *plan) {
+    if (!plan) return BCSR_OK;
+    int rc = hip_status(hipFree(plan->dev_base));
+    if (plan->d_ws) {
+        int rc2 = hip_status(hipFree(plan->d_ws));
+        if (rc == BCSR_OK) rc = rc2;
+    }
+    delete plan->ws_mu;
+    delete plan;
+    return rc;
+}
+
+int bcsr_plan_get_info(const bcsr_plan *plan, bcsr_plan_info *info) {
+    if (!plan || !info) return BCSR_ERR_INVALID_ARG;
+    *info = plan->info;
+    return BCSR_OK;
+}
+
+int bcsr_spmm(const bcsr_plan *plan, const void *B, float *C, int32_t N, void *stream) {
+    return launch_spmm(plan, B, C, N, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N, const bcsr_launch_cfg *cfg,
+                  void *stream) {
+    return launch_spmm(plan, B, C, N, cfg, static_cast<hipStream_t>(stream));
+}
+
+int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N, const bcsr_launch_cfg *cfg,
+                     int32_t reps, void *stream) {
+    if (reps < 0) return BCSR_ERR_INVALID_ARG;
+    for (int i = 0; i < reps; ++i) {
+        int rc = launch_spmm(plan, B, C, N, cfg, static_cast<hipStream_t>(stream));
+        if (rc != BCSR_OK) return rc;
+    }
+    return BCSR_OK;
+}
+
+int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg, char *buf,
+                          size_t buflen) {
+    if (!plan || !buf || buflen == 0) return BCSR_ERR_INVALID_ARG;
+    LaunchShape ls;
+    int rc = resolve_launch(plan, N, cfg, ls);
+    if (rc != BCSR_OK) return rc;
+    if (plan->info.precision == BCSR_PREC_FP8_E4M3)
+        fp8_kernel_name(plan, ls, buf, buflen);
+    else
+        pair_kernel_name(plan, ls, buf, buflen);
+    return BCSR_OK;
+}
+
+int bcsr_copy_to_device(const int32_t *hdr, const int32_t *idx, const void *data_f16, int32_t block_rows,
+                        int32_t block_cols, int32_t bh, int32_t bw, int **gpu_hdr, int **gpu_idx,
+                        void **gpu_data) {
+    if (!gpu_hdr || !gpu_idx || !gpu_data) return BCSR_ERR_INVALID_ARG;
+    bcsr_plan *plan = nullptr;
+    int rc = bcsr_plan_create(hdr, idx, data_f16, block_rows, block_cols, bh, bw, &plan);
+    if (rc != BCSR_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        registry()[plan->d_hdr] = plan;
+    }
+    *gpu_hdr = plan->d_hdr;
+    *gpu_idx = plan->d_idx;
+    *gpu_data = plan->d_vals;
+    return BCSR_OK;
+}
+
+int bcsr_release_device(const int *gpu_hdr) {
+    bcsr_plan *plan = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_reg_mu);
+        auto it = registry().find(gpu_hdr);
+        if (it == registry().end()) return BCSR_ERR_NOT_REGISTERED;
+        plan = it->second;
+        registry().erase(it);
+    }
+    return bcsr_plan_destroy(plan);
+}
+
+int bcsr_plan_from_device(const int *gpu_hdr, const bcsr_plan **out) {
+    if (!out) return BCSR_ERR_INVALID_ARG;
+    bcsr_plan *plan = nullptr;
+    int rc = lookup(gpu_hdr, 0, &plan);
+    *out = plan;
+    return rc;
+}
+
+#define BCSR_DEFINE_ENTRY(name, bh)                                                                  \
+    int name(const int *hdr, const int *idx, const void *data, const void *B, float *C, unsigned int M, \
+             unsigned int N, void *stream) {                                                          \
+        return entry(bh, hdr, idx, data, B, C, M, N, stream);                                         \
+    }
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor, 16)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor_v2, 16)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor_v3, 16)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor_v1_improved, 16)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor_option2_ldmatrix_sm80, 16)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor64x16, 64)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor64x16_v2, 64)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor32x16, 32)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor32x16_shared, 32)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor32x16_shared_vectorized, 32)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor32x16_v2, 32)
+BCSR_DEFINE_ENTRY(sparseMatrixMulTensor32x16_v2_shared_vectorized, 32)
+#undef BCSR_DEFINE_ENTRY
+
+int sparseMatrixMulTensor1(const int *hdr, const int *idx, const void *data, const void *B, float *C,
+                           unsigned int n, void *stream) {
+    return entry(16, hdr, idx, data, B, C, n, n, stream);
+}
+
+int sparseMatrixMulTensorAdaptive(const int *hdr, const int *idx, const void *data, const void *B, float *C,
+                                  unsigned int M, unsigned int N, unsigned int, void *stream) {
+    return entry(16, hdr, idx, data, B, C, M, N, stream);
+}
+
+int sparseMatrixMulTensorAdaptive32x16(const int *hdr, const int *idx, const void *data, const void *B,
+                                       float *C, unsigned int M, unsigned int N, unsigned int, void *stream) {
+    return entry(32, hdr, idx, data, B, C, M, N, stream);
+}
+
+int sparseMatrixMulTensorAdaptive64x16(const int *hdr, const int *idx, const void *data, const void *B,
+                                       float *C, unsigned int M, unsigned int N, unsigned int, void *stream) {
+    return entry(64, hdr, idx, data, B, C, M, N, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// C++ class device members
+// ---------------------------------------------------------------------------
+void BCSRMatrix::copyToDevice(int **gpuHdr, int **gpuIdx, half **gpuData) const {
+    void *d = nullptr;
+    int rc = bcsr_copy_to_device(hdr, idx, data, blockRows, blockCols, blockSizeRow, blockSizeCol, gpuHdr,
+                                 gpuIdx, &d);
+    if (rc != BCSR_OK)
+        throw std::runtime_error(std::string("BCSRMatrix::copyToDevice: ") + bcsr_status_string(rc));
+    *gpuData = static_cast<half *>(d);
+}
+
+void CSRMatrix::copyToDevice(int **gpuHdr, int **gpuIdx, half **gpuData) const {
+    const size_t nnz = static_cast<size_t>(hdr[N]);
+    auto fail = [](const char *what, hipError_t e) {
+        throw std::runtime_error(std::string("CSRMatrix::copyToDevice ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(reinterpret_cast<void **>(gpuData), sizeof(half) * (nnz ? nnz : 1))) != hipSuccess)
+        fail("malloc gpuData", e);
+    if ((e = hipMalloc(reinterpret_cast<void **>(gpuHdr), sizeof(int) * (N + 1))) != hipSuccess)
+        fail("malloc gpuHdr", e);
+    if ((e = hipMalloc(reinterpret_cast<void **>(gpuIdx), sizeof(int) * (nnz ? nnz : 1))) != hipSuccess)
+        fail("malloc gpuIdx", e);
+    if (nnz && (e = hipMemcpy(*gpuData, data, sizeof(half) * nnz, hipMemcpyHostToDevice)) != hipSuccess)
+        fail("gpuData", e);
+    if ((e = hipMemcpy(*gpuHdr, hdr, sizeof(int) * (N + 1), hipMemcpyHostToDevice)) != hipSuccess)
+        fail("gpuHdr", e);
+    if (nnz && (e = hipMemcpy(*gpuIdx, idx, sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess)
+        fail("gpuIdx", e);
+}

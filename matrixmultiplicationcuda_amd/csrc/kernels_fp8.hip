@@ -1,0 +1,220 @@
+// kernels_fp8.hip -- FP8 E4M3 storage mode on the scaled MFMA
+// v_mfma_scale_f32_16x16x128_f8f6f4 (no reference counterpart: the
+// reference's kernels are fp16 only).
+//
+// Every call first quantises B into a workspace (quant_b_fp8): one E8M0
+// scale per 16-row slab and 256-column chunk, and the values stored k-minor,
+// B8[slab][n][16] (the 16 rows of a slab for column n in 16 consecutive
+// bytes), so that 16 bytes are exactly one lane's half of an MFMA B operand
+// and no transpose is left for the product kernel.
+//
+// The product kernel walks quads of A blocks.  In one 16x16x128 MFMA each
+// 32-wide K group (lane group g of the wave) holds one block in its first 16
+// k and zeros in the second, so the MFMA's per-lane E8M0 operand scales are
+// exactly the per-block A scale and the per-slab B scale: the scaling costs no
+// VALU work.  Half of each MFMA's K is padding; the MFMA pipe is not the
+// bound (fp8 runs at twice the fp16 rate).
+#include <algorithm>
+#include <cstdio>
+
+#include "bcsr_internal.h"
+
+namespace {
+
+constexpr int kChunk = 256;  // columns per B scale
+
+// Smallest power of two 2^k (k in [-127, 127]) with absmax / 2^k <= 448, as
+// the E8M0 byte k + 127; 127 for an all-zero group.
+__device__ __forceinline__ unsigned e8m0_for_absmax(float amax) {
+    if (!(amax > 0.0f)) return 127u;
+    int e;
+    (void)frexpf(amax, &e);  // amax = m 2^e, m in [0.5, 1)
+    int k = (ldexpf(448.0f, e - 9) >= amax) ? e - 9 : e - 8;
+    k = k < -127 ? -127 : (k > 127 ? 127 : k);
+    return static_cast<unsigned>(k + 127);
+}
+
+// grid (K / 16, chunks), 256 threads: thread t quantises column
+// n = 256 * chunk + t of slab blockIdx.x.
+__global__ __launch_bounds__(256) void quant_b_fp8(const _Float16 *__restrict__ B, uint8_t *__restrict__ B8,
+                                                   uint8_t *__restrict__ scale, int N, int chunks) {
+    __shared__ float wmax[4];
+    const int c = blockIdx.x, h = blockIdx.y, t = threadIdx.x;
+    const int n = h * kChunk + t;
+    const bool valid = n < N;
+    float v[16];
+    float amax = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        v[r] = valid ? static_cast<float>(B[static_cast<size_t>(c * 16 + r) * N + n]) : 0.0f;
+        amax = fmaxf(amax, fabsf(v[r]));
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m, 64));
+    if ((t & 63) == 0) wmax[t >> 6] = amax;
+    __syncthreads();
+    amax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    const unsigned se = e8m0_for_absmax(amax);
+    if (t == 0) scale[static_cast<size_t>(c) * chunks + h] = static_cast<uint8_t>(se);
+    if (!valid) return;
+    const float inv = ldexpf(1.0f, 127 - static_cast<int>(se));
+    u32x4 q;
+    unsigned w[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        int p = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d] * inv, v[4 * d + 1] * inv, 0, false);
+        p = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d + 2] * inv, v[4 * d + 3] * inv, p, true);
+        w[d] = static_cast<unsigned>(p);
+    }
+    q = u32x4{w[0], w[1], w[2], w[3]};
+    *reinterpret_cast<u32x4 *>(B8 + (static_cast<size_t>(c) * N + n) * 16) = q;
+}
+
+// One wave per (device block-row, tile of 16 * EPL columns); MFMA e covers
+// columns tile + 16 e + j for lane j of each lane group.
+template <int RS, int EPL>
+__global__ __launch_bounds__(256) void bcsr_spmm_fp8(SpmmArgs a) {
+    int tile, slot;
+    if (!map_work(a, tile, slot)) return;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int br = a.row_order[slot];
+    const int n_base = tile * (16 * EPL);
+    const int chunk = n_base / kChunk;
+    const unsigned N = static_cast<unsigned>(a.N);
+    const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(a.B8, a.b_bytes);
+    const unsigned lane_off = static_cast<unsigned>(n_base + j) * 16u;
+
+    f32x4 acc[RS][EPL];
+#pragma unroll
+    for (int s = 0; s < RS; ++s)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
+    const u32x4 *vals = a.vals + lane;
+    for (int q = q0; q < q1; ++q) {
+        const int c = a.grp_col[4 * q + g];
+        const int sa = static_cast<int>((a.ascale[q] >> (8 * g)) & 0xffu);
+        const int sb = a.b8scale[static_cast<size_t>(c) * a.b_chunks + chunk];
+        const unsigned off = static_cast<unsigned>(c) * (16u * N) + lane_off;
+        u32x4 bq[EPL];
+#pragma unroll
+        for (int e = 0; e < EPL; ++e)
+            bq[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off + 256u * e), 0, 0);
+        i32x8 av[RS];
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+            const u32x4 x = vals[(static_cast<size_t>(q) * RS + s) * 64];
+            av[s] = i32x8{static_cast<int>(x.x), static_cast<int>(x.y), static_cast<int>(x.z), static_cast<int>(x.w),
+                          0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+            const i32x8 bv = i32x8{static_cast<int>(bq[e].x), static_cast<int>(bq[e].y), static_cast<int>(bq[e].z),
+                                   static_cast<int>(bq[e].w), 0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < RS; ++s)
+                acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av[s], bv, acc[s][e], 0, 0, 0, sa, 0, sb);
+        }
+    }
+
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+        const int n = n_base + 16 * e + j;
+        if (n >= a.N) continue;
+#pragma unroll
+        for (int s = 0; s < RS; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t row = static_cast<size_t>(br) * (16 * RS) + 16 * s + 4 * g + r;
+                a.C[row * static_cast<size_t>(a.N) + n] = acc[s][e][r];
+            }
+    }
+}
+
+typedef void (*Kernel)(SpmmArgs);
+
+template <int RS>
+Kernel pick_epl(int epl) {
+    switch (epl) {
+        case 2: return bcsr_spmm_fp8<RS, 2>;
+        case 4: return bcsr_spmm_fp8<RS, 4>;
+        case 8: return bcsr_spmm_fp8<RS, 8>;
+        case 16: return bcsr_spmm_fp8<RS, 16>;
+        default: return nullptr;
+    }
+}
+
+Kernel pick_kernel(int rs, int epl) {
+    switch (rs) {
+        case 1: return pick_epl<1>(epl);
+        case 2: return pick_epl<2>(epl);
+        case 4: return pick_epl<4>(epl);
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len) {
+    std::snprintf(buf, len, "bcsr_spmm_fp8<%d, %d>", plan->rs, ls.epl);
+}
+
+int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream) {
+    Kernel k = pick_kernel(plan->rs, ls.epl);
+    if (!k) return BCSR_ERR_UNSUPPORTED;
+    const int K = plan->info.cols;
+    const int slabs = K / 16;
+    const int chunks = (N + kChunk - 1) / kChunk;
+    const size_t q_bytes = static_cast<size_t>(K) * N;
+    const size_t need = (q_bytes + 255) / 256 * 256 + static_cast<size_t>(slabs) * chunks;
+    std::lock_guard<std::mutex> lk(*plan->ws_mu);
+    if (plan->ws_bytes < need) {
+        if (plan->d_ws) {
+            int rc = hip_status(hipFree(plan->d_ws));
+            plan->d_ws = nullptr;
+            plan->ws_bytes = 0;
+            if (rc != BCSR_OK) return rc;
+        }
+        int rc = hip_status(hipMalloc(&plan->d_ws, need));
+        if (rc != BCSR_OK) return rc;
+        plan->ws_bytes = need;
+    }
+    uint8_t *B8 = static_cast<uint8_t *>(plan->d_ws);
+    uint8_t *bscale = B8 + (q_bytes + 255) / 256 * 256;
+    hipLaunchKernelGGL(quant_b_fp8, dim3(slabs, chunks), dim3(256), 0, stream, static_cast<const _Float16 *>(B), B8,
+                       bscale, N, chunks);
+    int rc = hip_status(hipGetLastError());
+    if (rc != BCSR_OK) return rc;
+    SpmmArgs a{};
+    a.grp_ptr = plan->d_grp_ptr;
+    a.grp_col = plan->d_grp_col;
+    a.vals = static_cast<const u32x4 *>(plan->d_vals);
+    a.ascale = plan->d_ascale;
+    a.row_order = plan->d_row_order;
+    a.B8 = B8;
+    a.b8scale = bscale;
+    a.C = C;
+    a.N = N;
+    a.n_tiles = ls.n_tiles;
+    a.dev_rows = plan->dev_rows;
+    a.map_mode = ls.map_mode;
+    a.b_bytes = static_cast<int>(q_bytes);
+    a.b_chunks = chunks;
+    a.wpb = ls.wpb;
+    hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *stream) {
+    if (K < 0 || N < 0 || K % 16 || N % 16) return BCSR_ERR_SHAPE;
+    if (!K || !N) return BCSR_OK;
+    if (!B || !B8 || !scales) return BCSR_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(B8) & 15u) || (reinterpret_cast<uintptr_t>(B) & 1u)) return BCSR_ERR_SHAPE;
+    const int chunks = (N + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(quant_b_fp8, dim3(K / 16, chunks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const _Float16 *>(B), static_cast<uint8_t *>(B8), static_cast<uint8_t *>(scales), N,
+                       chunks);
+    return hip_status(hipGetLastError());
+}

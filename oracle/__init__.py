@@ -61,6 +61,15 @@ def lib():
         f("orc_matmul_dense_ref", None, _P, _P, i, i, i, _P)
         f("orc_bcsr_spmm_f32", None, _P, _P, _P, i, i, i, i, _P, i, _P, i)
         f("orc_bcsr_spmm_f64", None, _P, _P, _P, i, i, i, i, _P, i, _P, _P, i)
+        f("orc_f2e4m3", ctypes.c_uint8, ctypes.c_float)
+        f("orc_e4m32f", ctypes.c_float, ctypes.c_uint8)
+        f("orc_e8m0_for_absmax", ctypes.c_uint8, ctypes.c_float)
+        f("orc_quant_e4m3_groups", None, _P, ctypes.c_int64, ctypes.c_int64, _P, _P)
+        f("orc_quant_e4m3_B", None, _P, i, i, _P, _P)
+        f("orc_f2bf16", ctypes.c_uint16, ctypes.c_float)
+        f("orc_bf162f", ctypes.c_float, ctypes.c_uint16)
+        f("orc_hilo_split", None, ctypes.c_float, ctypes.POINTER(ctypes.c_uint16),
+          ctypes.POINTER(ctypes.c_uint16))
         _o = o
     return _o
 
@@ -235,3 +244,96 @@ def bcsr_spmm_f64(hdr, idx, data32, bh, bw, Bf32, br0=0, br1=None, threads=1, wa
     lib().orc_bcsr_spmm_f64(_p(hdr), _p(idx), _p(data), bh, bw, br0, br1, _p(Bf), N, _p(C),
                             _p(Ca) if want_abs else None, threads)
     return C, Ca
+
+
+# ---- FP8 E4M3 storage mode ------------------------------------------------
+def f32_to_e4m3(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return np.array([lib().orc_f2e4m3(float(v)) for v in x.ravel()], dtype=np.uint8).reshape(x.shape)
+
+
+E4M3_TABLE = None
+
+
+def e4m3_to_f32(q):
+    global E4M3_TABLE
+    if E4M3_TABLE is None:
+        E4M3_TABLE = np.array([lib().orc_e4m32f(b) for b in range(256)], dtype=np.float32)
+    return E4M3_TABLE[np.asarray(q, dtype=np.uint8)]
+
+
+def quant_e4m3_groups(x, group_len):
+    """(q, scale_exp): one E8M0 scale per consecutive group of group_len."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.size // group_len
+    q = np.empty(x.shape, dtype=np.uint8)
+    sc = np.empty(n, dtype=np.uint8)
+    lib().orc_quant_e4m3_groups(_p(x), n, group_len, _p(q), _p(sc))
+    return q, sc
+
+
+def quant_e4m3_B(B):
+    """(q K x N, scale_exp [K/16, ceil(N/256)]) of the B quantiser."""
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    K, N = B.shape
+    q = np.empty((K, N), dtype=np.uint8)
+    sc = np.empty((K // 16, (N + 255) // 256), dtype=np.uint8)
+    lib().orc_quant_e4m3_B(_p(B), K, N, _p(q), _p(sc))
+    return q, sc
+
+
+def e8m0_to_f32(e):
+    return np.ldexp(np.float32(1.0), np.asarray(e, dtype=np.int32) - 127).astype(np.float32)
+
+
+def f32_to_bf16_bits(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return np.array([lib().orc_f2bf16(float(v)) for v in x.ravel()],
+                    dtype=np.uint16).reshape(x.shape)
+
+
+def bf16_bits_to_f32(b):
+    return (np.asarray(b, dtype=np.uint32) << 16).astype(np.uint32).view(np.float32)
+
+
+def hilo_split(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    hi = np.empty(x.shape, np.uint16)
+    lo = np.empty(x.shape, np.uint16)
+    h, l_ = ctypes.c_uint16(), ctypes.c_uint16()
+    for k, v in enumerate(x.ravel()):
+        lib().orc_hilo_split(float(v), ctypes.byref(h), ctypes.byref(l_))
+        hi.flat[k], lo.flat[k] = h.value, l_.value
+    return hi, lo
+
+
+def dequantized_blocks(mode, data32, bh, bw):
+    """Block values as the storage mode keeps them (fp32-exact)."""
+    data32 = np.ascontiguousarray(data32, dtype=np.float32)
+    if mode == "f16":
+        return f16_bits_to_f32(f32_to_f16_bits(data32))
+    if mode == "bf16":
+        return bf16_bits_to_f32(f32_to_bf16_bits(data32))
+    if mode == "hilo":
+        hi, lo = hilo_split(data32)
+        return (f16_bits_to_f32(hi).astype(np.float64)
+                + f16_bits_to_f32(lo).astype(np.float64) * 2.0**-11)
+    if mode == "fp8":
+        q, sc = quant_e4m3_groups(data32, bh * bw)
+        return (e4m3_to_f32(q).reshape(-1, bh * bw) * e8m0_to_f32(sc)[:, None]).reshape(data32.shape)
+    raise ValueError(mode)
+
+
+def quantized_B(mode, B16_bits):
+    """B (given in fp16) as the mode's kernel multiplies it."""
+    Bf = f16_bits_to_f32(B16_bits)
+    if mode in ("f16", "hilo"):
+        return Bf
+    if mode == "bf16":
+        return bf16_bits_to_f32(f32_to_bf16_bits(Bf))
+    if mode == "fp8":
+        q, sc = quant_e4m3_B(Bf)
+        K, N = Bf.shape
+        scale = np.repeat(np.repeat(e8m0_to_f32(sc), 16, axis=0), 256, axis=1)[:K, :N]
+        return e4m3_to_f32(q) * scale
+    raise ValueError(mode)

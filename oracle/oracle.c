@@ -541,3 +541,97 @@ void orc_bcsr_spmm_f64(const int32_t *hdr, const int32_t *idx, const float *data
     jb.Cabs = Cabs;
     orc_run_split(jb, nthreads, orc_spmm_f64_thread);
 }
+
+/* ---------------------------------------------------------------------------
+ * FP8 E4M3 quantisation (storage-mode extension; no reference counterpart).
+ * ------------------------------------------------------------------------ */
+uint8_t orc_f2e4m3(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint8_t sign = (uint8_t)((x >> 24) & 0x80u);
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return sign | 0x7fu;  /* NaN */
+    if (ax >= 0x43e00000u) return sign | 0x7eu; /* >= 448 (and inf): saturate */
+    if (ax < 0x3c800000u) {                     /* below 2^-6: multiples of 2^-9 */
+        uint32_t e = ax >> 23;
+        if (e < 116u) return sign;              /* below 2^-11: rounds to 0 */
+        uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        uint32_t shift = 141u - e;              /* value / 2^-9 = m * 2^(e - 141) */
+        uint32_t r = m >> shift;
+        uint32_t rem = m & ((1u << shift) - 1u);
+        uint32_t halfway = 1u << (shift - 1u);
+        if (rem > halfway || (rem == halfway && (r & 1u))) r++;
+        return sign | (uint8_t)r;               /* r == 8 is the smallest normal */
+    }
+    uint32_t e = ax >> 23;
+    uint32_t r = ((e - 120u) << 3) | ((ax >> 20) & 7u);
+    uint32_t rem = ax & 0xfffffu;
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) r++;
+    if (r > 0x7eu) r = 0x7eu;
+    return sign | (uint8_t)r;
+}
+
+float orc_e4m32f(uint8_t b) {
+    int e = (b >> 3) & 15, m = b & 7;
+    float v;
+    if (e == 15 && m == 7) return NAN;
+    if (e == 0)
+        v = ldexpf((float)m, -9);
+    else
+        v = ldexpf(1.0f + (float)m / 8.0f, e - 7);
+    return (b & 0x80) ? -v : v;
+}
+
+uint8_t orc_e8m0_for_absmax(float absmax) {
+    if (!(absmax > 0.0f)) return 127;
+    int k = -127;
+    while (k < 127 && ldexpf(448.0f, k) < absmax) k++;
+    return (uint8_t)(k + 127);
+}
+
+void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, uint8_t *scale) {
+    for (int64_t g = 0; g < n; ++g) {
+        float amax = 0.0f;
+        for (int64_t i = 0; i < len; ++i) amax = fmaxf(amax, fabsf(x[g * len + i]));
+        uint8_t se = orc_e8m0_for_absmax(amax);
+        scale[g] = se;
+        for (int64_t i = 0; i < len; ++i) q[g * len + i] = orc_f2e4m3(ldexpf(x[g * len + i], 127 - (int)se));
+    }
+}
+
+void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale) {
+    int chunks = (N + 255) / 256;
+    for (int c = 0; c < K / 16; ++c)
+        for (int h = 0; h < chunks; ++h) {
+            int n0 = h * 256, n1 = n0 + 256 < N ? n0 + 256 : N;
+            float amax = 0.0f;
+            for (int r = 0; r < 16; ++r)
+                for (int n = n0; n < n1; ++n) amax = fmaxf(amax, fabsf(B[(size_t)(c * 16 + r) * N + n]));
+            uint8_t se = orc_e8m0_for_absmax(amax);
+            scale[(size_t)c * chunks + h] = se;
+            for (int r = 0; r < 16; ++r)
+                for (int n = n0; n < n1; ++n)
+                    q[(size_t)(c * 16 + r) * N + n] = orc_f2e4m3(ldexpf(B[(size_t)(c * 16 + r) * N + n], 127 - (int)se));
+        }
+}
+
+/* bfloat16 and hi+lo fp16 storage (extensions). */
+uint16_t orc_f2bf16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+    uint32_t lsb = (x >> 16) & 1u;
+    return (uint16_t)((x + 0x7fffu + lsb) >> 16);
+}
+
+float orc_bf162f(uint16_t b) {
+    uint32_t x = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+void orc_hilo_split(float a, uint16_t *hi, uint16_t *lo) {
+    *hi = orc_f2h(a);
+    *lo = orc_f2h((a - orc_h2f(*hi)) * 2048.0f);
+}

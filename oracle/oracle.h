@@ -133,6 +133,31 @@ void orc_bcsr_spmm_f64(const int32_t *hdr, const int32_t *idx, const float *data
                        int bh, int bw, int br0, int br1, const float *Bf, int N,
                        double *C, double *Cabs, int nthreads);
 
+/* ---- FP8 E4M3 (OCP "FN": bias 7, max 448, no infinity) -----------------
+ * Extension of the reference (which is fp16 only): the quantiser of the FP8
+ * storage mode.  Round to nearest even, finite overflow saturates to +-448.
+ * Scales are powers of two stored as E8M0 bytes (2^(e - 127)): the smallest
+ * 2^k with absmax / 2^k <= 448, 127 for an all-zero group.
+ * pinned: exhaustive decode table vs the OCP definition, round trip of all
+ * 254 finite codes, midpoints (ties to even), saturation; see tests. */
+uint8_t orc_f2e4m3(float f);
+float orc_e4m32f(uint8_t b);
+uint8_t orc_e8m0_for_absmax(float absmax);
+/* Quantise n groups of `len` floats, one E8M0 scale per group. */
+void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, uint8_t *scale);
+/* Quantise B (K x N fp32, row-major): one scale per 16-row slab and
+ * 256-column chunk, q row-major like B. */
+void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale);
+
+/* ---- other storage modes (extensions) ---------------------------------
+ * bf16: float -> bfloat16 round to nearest even.  hi+lo: hi = fp16(a),
+ * lo = fp16((a - hi) * 2^11), a ~ hi + 2^-11 lo.  pinned: bf16 vs numpy's
+ * float32 bit manipulation identity on all halves; hi+lo by the exactness of
+ * a - hi (checked in tests). */
+uint16_t orc_f2bf16(float f);
+float orc_bf162f(uint16_t b);
+void orc_hilo_split(float a, uint16_t *hi, uint16_t *lo);
+
 #ifdef __cplusplus
 }
 #endif

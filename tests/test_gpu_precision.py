@@ -1,0 +1,123 @@
+"""GPU parity of the storage-precision modes (bf16, hi+lo fp16, FP8 E4M3).
+
+These modes have no reference counterpart (the reference is fp16 only).  Each
+is checked twice:
+  1. against the exact product of the values the mode stores (the oracle's
+     restatement of the quantisers, dequantised, multiplied in fp64): the
+     kernel may differ only by fp32 accumulation, bound 1e-4 * (|Aq|.|Bq|)_ij;
+  2. against the FP64 product of the unquantised A (fp32 generator values)
+     and B (the fp16 input), with each mode's error bound per entry relative
+     to (|A|.|B|)_ij: fp16 2^-11, bf16 2^-8 + 2^-8 (A and the B operand are
+     rounded), hi+lo 5e-5; FP8 (2^-4 per operand above E4M3's subnormal
+     range) mean <= 2e-2 and max <= 0.25 on log-uniform values.
+The measured maxima are printed (config-2 reports them per mode).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import matrixmultiplicationcuda_amd as mm
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+BOUND_VS_FP64 = {"f16": 2.0**-11 + 1e-6, "bf16": 2.0**-8 * 2 + 2.0**-16 + 1e-6, "hilo": 5e-5}
+PATTERN = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def case(M, K, N, bh, sparsity=0.9, dist="loguniform", seed=321):
+    A32 = mm.generate(M, K, sparsity, PATTERN[bh], bh=bh, bw=bh, seed=seed, value_dist=dist,
+                      lo=1e-3, hi=1e2, dtype=np.float32)
+    h = mm.bcsr_from_dense(A32.astype(np.float16), bh, 16)
+    data32 = oracle.bcsr_values_f32(A32, h.hdr, h.idx, bh, 16)
+    h32 = mm.HostBCSR(h.hdr, h.idx, data32, h.block_rows, h.block_cols, bh, 16)
+    B16 = mm.generate(K, N, 0.0, "random", bh=16, seed=456)
+    return h32, data32, B16
+
+
+def run_mode(h32, B16, mode, cfg=None):
+    assert mm.device_count() > 0
+    plan = mm.Plan(h32, mode)
+    C = plan.matmul(B16, cfg=cfg)
+    plan.close()
+    return C
+
+
+@pytest.mark.parametrize("mode", ["f16", "bf16", "hilo", "fp8"])
+@pytest.mark.parametrize("bh,N", [(16, 256), (16, 96), (32, 128), (64, 64), (16, 512)])
+def test_precision_mode(mode, bh, N):
+    M, K = 512, 1024
+    h32, data32, B16 = case(M, K, N, bh)
+    log(f"mode={mode} bh={bh} N={N} nnzb={h32.nnzb}")
+    C = run_mode(h32, B16, mode).astype(np.float64)
+    Aq = oracle.dequantized_blocks(mode, data32, bh, 16)
+    Aq32 = np.asarray(Aq, dtype=np.float32)
+    assert np.array_equal(Aq32.astype(np.float64), np.asarray(Aq, dtype=np.float64))
+    Bq = oracle.quantized_B(mode, B16.view(np.uint16))
+    Cq, Cqabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, Aq32, bh, 16, Bq, threads=8)
+    err_q = np.abs(C - Cq)
+    assert np.all(err_q <= 1e-4 * Cqabs + 1e-30), f"{mode}: kernel differs from its stored values"
+    Bf = B16.astype(np.float32)
+    C64, Cabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, data32, bh, 16, Bf, threads=8)
+    nz = Cabs > 0
+    rel = np.abs(C - C64)[nz] / Cabs[nz]
+    normwise = np.linalg.norm(C - C64) / np.linalg.norm(C64)
+    log(f"  {mode}: max rel |A||B| {rel.max():.3e} mean {rel.mean():.3e} normwise {normwise:.3e}")
+    if mode == "fp8":
+        # E4M3 keeps 2^-4 relative precision only above its subnormal range,
+        # which log-uniform blocks over 5 decades reach: report, bound loosely
+        assert rel.mean() <= 2e-2 and rel.max() <= 0.25, f"fp8 error vs FP64 {rel.max():.3e}"
+    else:
+        assert rel.max() <= BOUND_VS_FP64[mode], f"{mode}: error vs FP64 above bound ({rel.max():.3e})"
+    assert np.all(C[~nz] == 0)
+
+
+def test_fp8_b_quantiser_matches_oracle():
+    """bcsr_quantize_b_e4m3 == the oracle's quantiser, byte for byte."""
+    lib = mm.lib()
+    K, N = 256, 640  # 3 column chunks, the last partial
+    B16 = mm.generate(K, N, 0.0, "random", seed=11)
+    B16[5, 7] = np.float16(0.0)
+    B16[17, :] *= np.float16(1e-3)  # a slab with small values (subnormal codes)
+    dB = mm.DeviceArray.from_numpy(B16)
+    d8 = mm.DeviceArray((K * N,), np.uint8)
+    chunks = (N + 255) // 256
+    ds = mm.DeviceArray((K // 16 * chunks,), np.uint8)
+    mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.ptr, None))
+    mm.synchronize()
+    q_k_minor = d8.numpy().reshape(K // 16, N, 16)
+    got = q_k_minor.transpose(0, 2, 1).reshape(K, N)
+    want_q, want_s = oracle.quant_e4m3_B(B16.astype(np.float32))
+    assert np.array_equal(ds.numpy().reshape(K // 16, chunks), want_s)
+    assert np.array_equal(got, want_q)
+
+
+@pytest.mark.parametrize("mode", ["bf16", "hilo", "fp8"])
+def test_precision_uniform_values_target_shape_rows(mode):
+    """16384^2, 5% blocks, N = 512 (U(-1,1) values): sampled block-rows."""
+    M = K = 16384
+    N = 512
+    A32 = mm.generate(M, K, 0.95, "blockrandom", bh=16, seed=123, dtype=np.float32)
+    h = mm.bcsr_from_dense(A32.astype(np.float16), 16, 16)
+    data32 = oracle.bcsr_values_f32(A32, h.hdr, h.idx, 16, 16)
+    del A32
+    h32 = mm.HostBCSR(h.hdr, h.idx, data32, h.block_rows, h.block_cols, 16, 16)
+    B16 = mm.generate(K, N, 0.0, "random", seed=456)
+    C = run_mode(h32, B16, mode).astype(np.float64)
+    Bf = B16.astype(np.float32)
+    rows = list(range(0, 1024, 37))
+    worst = 0.0
+    for br in rows:
+        C64, Cabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, data32, 16, 16, Bf, br, br + 1)
+        sl = slice(br * 16, br * 16 + 16)
+        nz = Cabs[sl] > 0
+        rel = (np.abs(C[sl] - C64[sl])[nz] / Cabs[sl][nz]).max(initial=0.0)
+        worst = max(worst, rel)
+    log(f"  {mode} target rows: max rel |A||B| {worst:.3e}")
+    assert worst <= {"bf16": 2e-2, "hilo": 5e-5, "fp8": 2e-2}[mode]

@@ -7,6 +7,7 @@ where (|A|.|B|)_ij = 0, plus the reference's own |diff| <= 1e-2
 (nvbench_main.cu:413-420).
 """
 import ctypes
+import functools
 import sys
 
 import numpy as np
@@ -67,16 +68,33 @@ def test_block_shapes_and_widths(bh, pattern, N):
     assert_close(C, C_cpu, Cabs, what=f"bh={bh} N={N}")
 
 
+@functools.lru_cache(maxsize=None)
+def launch_case(bh, long_rows):
+    """1024 x 2048, N = 512; long_rows (sparsity 0.3) gives block-rows of
+    ~45-90 pairs, exercising long pipelines and odd tails."""
+    pattern = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}[bh]
+    case = make_case(1024, 2048, 512, 0.3 if long_rows else 0.85, pattern, bh, 16)
+    C_cpu, Cabs = cpu_reference(case)
+    return case, C_cpu, Cabs
+
+
+@pytest.mark.parametrize("wpb", [1, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("tile_n", [32, 64, 128, 256])
 @pytest.mark.parametrize("xcd_map", [0, 1])
-@pytest.mark.parametrize("bh", [16, 64])
-def test_launch_configs(tile_n, xcd_map, bh):
-    N = 512
-    case = make_case(1024, 1024, N, 0.85, "blockrandom" if bh == 16 else "pattern64by16", bh, 16)
-    log(f"cfg tile={tile_n} map={xcd_map} bh={bh}")
-    C = run_plan(case, cfg=mm.make_cfg(tile_n=tile_n, xcd_map=xcd_map))
-    C_cpu, Cabs = cpu_reference(case)
-    assert_close(C, C_cpu, Cabs, what=f"tile={tile_n} map={xcd_map}")
+@pytest.mark.parametrize("bh", [16, 32, 64])
+def test_launch_configs(tile_n, xcd_map, bh, variant, wpb):
+    case, C_cpu, Cabs = launch_case(bh, variant > 0)
+    log(f"cfg tile={tile_n} map={xcd_map} bh={bh} variant={variant} wpb={wpb}")
+    rs, epl = bh // 16, tile_n // 16
+    if (epl == 16 and rs != 1) or (variant == 1 and rs * epl > 16):
+        # shapes whose register tiles do not fit are not instantiated
+        with pytest.raises(mm.BcsrError) as e:
+            run_plan(case, cfg=mm.make_cfg(tile_n, xcd_map, variant, wpb))
+        assert e.value.status == -7
+        return
+    C = run_plan(case, cfg=mm.make_cfg(tile_n, xcd_map, variant, wpb))
+    assert_close(C, C_cpu, Cabs, what=f"tile={tile_n} map={xcd_map} variant={variant} wpb={wpb}")
 
 
 @pytest.mark.parametrize("pattern", mm.REFERENCE_PATTERNS + mm.EXTRA_PATTERNS)
