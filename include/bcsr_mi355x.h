@@ -192,6 +192,15 @@ int mg_generate_f32(int32_t rows, int32_t cols, double sparsity, const char *pat
 int mg_generate_f16(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,
                     int32_t bw, int32_t band_halfwidth, double inblock_density,
                     int32_t value_dist, double lo, double hi, uint32_t seed, void *out);
+/* mg::generate_bcsr: the BCSR form (bh x bw blocks) of mg_generate_f16's
+ * matrix for the block patterns, generated block by block.  hdr must hold
+ * rows / bh + 1 entries; *idx and *data (nnzb and nnzb * bh * bw floats) are
+ * allocated by the library and released with mg_free. */
+int mg_generate_bcsr(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,
+                     int32_t bw, int32_t band_halfwidth, double inblock_density, int32_t value_dist,
+                     double lo, double hi, uint32_t seed, int32_t *hdr, int64_t *nnzb,
+                     int32_t **idx, float **data);
+void mg_free(void *p);
 /* BCSRMatrix construction from dense fp16 (formats/BCSRMatrix.cu:22-74) in
  * two calls: count (fills hdr[rows/bh + 1], returns nnzb) then fill. */
 int bcsr_host_count(const void *dense_f16, int32_t rows, int32_t cols, int32_t bh, int32_t bw,

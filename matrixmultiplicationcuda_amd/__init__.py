@@ -128,6 +128,10 @@ def _declare(lib):
     for name in ("mg_generate_f32", "mg_generate_f16"):
         f(name, i, _I32, _I32, ctypes.c_double, ctypes.c_char_p, _I32, _I32, _I32, ctypes.c_double,
           _I32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _P)
+    f("mg_generate_bcsr", i, _I32, _I32, ctypes.c_double, ctypes.c_char_p, _I32, _I32, _I32,
+      ctypes.c_double, _I32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _P, _I64P,
+      ctypes.POINTER(_P), ctypes.POINTER(_P))
+    f("mg_free", None, _P)
     f("bcsr_host_count", i, _P, _I32, _I32, _I32, _I32, _P, _I64P)
     f("bcsr_host_fill", i, _P, _I32, _I32, _I32, _I32, _P, _P, _P)
     f("mat_read_header", i, ctypes.c_char_p, _I32P, _I32P)
@@ -224,6 +228,30 @@ class HostBCSR:
     @property
     def shape(self):
         return self.block_rows * self.bh, self.block_cols * self.bw
+
+
+def generate_bcsr(rows, cols, sparsity, pattern, bh=16, bw=16, seed=123, band_halfwidth=8,
+                  inblock_density=1.0, value_dist="uniform", lo=1e-3, hi=1e2) -> HostBCSR:
+    """BCSR (float32 block values) of generate(...) for block patterns, built
+    block by block without the dense matrix (mg::generate_bcsr)."""
+    vd = {"uniform": 0, "loguniform": 1}[value_dist]
+    hdr = np.zeros(rows // bh + 1, dtype=np.int32)
+    nnzb = ctypes.c_int64(0)
+    pidx, pdata = ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().mg_generate_bcsr(rows, cols, float(sparsity), pattern.encode(), bh, bw,
+                                 band_halfwidth, float(inblock_density), vd, float(lo), float(hi),
+                                 seed, _ptr(hdr), ctypes.byref(nnzb), ctypes.byref(pidx),
+                                 ctypes.byref(pdata)), f"generate_bcsr({pattern})")
+    n = nnzb.value
+    try:
+        idx = np.ctypeslib.as_array(ctypes.cast(pidx, ctypes.POINTER(ctypes.c_int32)),
+                                    shape=(max(n, 1),))[:n].copy()
+        data = np.ctypeslib.as_array(ctypes.cast(pdata, ctypes.POINTER(ctypes.c_float)),
+                                     shape=(max(n, 1) * bh * bw,))[:n * bh * bw].copy()
+    finally:
+        lib().mg_free(pidx)
+        lib().mg_free(pdata)
+    return HostBCSR(hdr, idx, data.reshape(n, bh, bw), rows // bh, cols // bw, bh, bw)
 
 
 def bcsr_from_dense(A: np.ndarray, bh=16, bw=16) -> HostBCSR:

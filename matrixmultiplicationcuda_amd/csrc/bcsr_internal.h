@@ -15,10 +15,13 @@
 //                              = rows 16 s + l % 16, columns 8 (l / 16) .. + 7
 //                              of the pair's 16 x 32 slice (hi+lo mode stores
 //                              [group][s][hi, lo][lane]);
-//                            quads: [group][s][lane l][16 x fp8] = row
-//                              16 s + l % 16 of slot l / 16's block;
+//                            quads: [group][s][lane l < 32][32 x fp8]: bytes
+//                              16 h .. 16 h + 15 of lane l = row 16 s + l % 16
+//                              of the block in slot 2 h + l / 16 (the MFMA's
+//                              operand for lane group 2 (l / 16); lane groups
+//                              1 and 3 hold zeros and are not stored);
 //     ascale                 FP8 only: per group the four slots' E8M0 block
-//                            scales, byte g = slot g;
+//                            scales, byte t = slot t;
 //     row_order[dev_rows]    device block-rows by descending group count.
 #pragma once
 
@@ -36,6 +39,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x8 __attribute__((ext_vector_type(8)));
 
 struct bcsr_plan {
     bcsr_plan_info info{};
@@ -84,6 +88,7 @@ struct SpmmArgs {
     int b_bytes;  // bytes of B (or B8) when below 2^31: buffer-load bound
     int b_chunks;  // 256-column chunks of B8 scales
     int wpb;       // waves per workgroup
+    int a_bytes;   // bytes of the FP8 A operands (buffer-load bound)
 };
 
 int hip_status(hipError_t e);
@@ -121,6 +126,12 @@ __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot
     }
     return slot < a.dev_rows;
 }
+
+// Offsets at or above this are out of range for every buffer (num_records is
+// at most INT32_MAX): a load there returns zeros without touching memory.
+// Only voffset (plus the instruction offset) is range-checked, so kernels keep
+// soffset strides for loads already known to be in bounds.
+constexpr unsigned kOutOfRange = 0x80000000u;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
     // gfx9 buffer descriptor word 3; num_records = bytes (reads past it return 0)

@@ -159,6 +159,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     for (int r = 0; r < dev_rows; ++r) grp_ptr[r + 1] += grp_ptr[r];
     const int64_t groups = grp_ptr[dev_rows];
     const size_t chunk_bytes = 16;                                 // per lane per operand
+    // pairs: 64 lanes x 16 bytes per operand; quads: 32 lanes x 32 bytes
     const size_t group_bytes = static_cast<size_t>(rs) * H * 64 * chunk_bytes;
 
     std::vector<int32_t> grp_col(static_cast<size_t>(G) * groups);
@@ -185,8 +186,10 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
                             const int g = lane >> 4, i = lane & 15;
                             const int row = t * dev_bh + 16 * s + i;
                             if (fp8) {
-                                if (g != slot) continue;
-                                uint8_t *dst = gv + (static_cast<size_t>(s) * 64 + lane) * chunk_bytes;
+                                // quad slot -> 16 bytes of lane (16 (slot & 1) + i) at
+                                // half slot >> 1; lanes are stored compactly (32 of them)
+                                if (lane >= 32 || g != (slot & 1)) continue;
+                                uint8_t *dst = gv + (static_cast<size_t>(s) * 32 + lane) * 32 + 16 * (slot >> 1);
                                 for (int e = 0; e < 16; ++e) {
                                     const int64_t si = k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + e;
                                     dst[e] = bcsr_f32_to_e4m3(src.value(si) * inv);
@@ -344,17 +347,21 @@ int entry(int bh, const int *hdr, const int *idx, const void *data, const void *
 }  // namespace
 
 int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls) {
-    int tile_n = cfg ? cfg->tile_n : 0;
-    if (tile_n == 0) tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
-    if (cfg && cfg->tile_n == 0 && plan->rs == 4 && tile_n == 128 && plan->info.precision == BCSR_PREC_F16_HILO)
-        tile_n = 64;
-    if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
-    // automatic choice: the prefetching kernel for 16-row device blocks, the
-    // plain one for 32/64-row blocks (whose larger register tiles leave no room
-    // for a second set of loads)
     const bool fp8 = plan->info.precision == BCSR_PREC_FP8_E4M3;
+    int tile_n = cfg ? cfg->tile_n : 0;
+    if (tile_n == 0) {
+        // fastest measured on MI355X at 16384 x 16384, 5% blocks, N = 512:
+        // 128-column tiles for 16/32-row device blocks, 64 for 64-row blocks,
+        // 256 for FP8
+        tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
+        if (plan->rs == 4 && tile_n == 128) tile_n = 64;
+        if (fp8 && N >= 256) tile_n = 256;
+    }
+    if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
+    // automatic choice: the software-pipelined kernel where its register
+    // footprint fits, else the plain one
     const bool auto_variant = !cfg || cfg->variant < 0;
-    ls.variant = auto_variant ? (plan->rs == 1 ? 1 : 0) : cfg->variant;
+    ls.variant = auto_variant ? 1 : cfg->variant;
     if (ls.variant > 2) return BCSR_ERR_UNSUPPORTED;
     if (!fp8 && !pair_shape_supported(plan->info.precision, plan->rs, tile_n / 16, ls.variant)) {
         if (!auto_variant || ls.variant != 1) return BCSR_ERR_UNSUPPORTED;
@@ -364,7 +371,9 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     if (fp8 && ls.variant == 2) return BCSR_ERR_UNSUPPORTED;
     // buffer offsets are 32-bit: larger B falls back to 64-bit global loads
     if (!fp8 && static_cast<int64_t>(plan->info.cols) * N * 2 > INT32_MAX) ls.variant = 2;
-    if (fp8 && static_cast<int64_t>(plan->info.cols) * N > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
+    if (fp8 && (static_cast<int64_t>(plan->info.cols) * N > INT32_MAX ||
+                plan->info.dev_pairs * plan->rs * 1024 > INT32_MAX))
+        return BCSR_ERR_UNSUPPORTED;
     ls.epl = tile_n / 16;
     ls.n_tiles = (N + tile_n - 1) / tile_n;
     const bool xcd_ok = ls.n_tiles == 1 || ls.n_tiles == 2 || ls.n_tiles == 4 || ls.n_tiles % 8 == 0;

@@ -1,5 +1,7 @@
 // capi_host.cpp -- C ABI over the host-side generator
 // (mg::generate_matrix<T>, matrix_generator.cpp:28-140, and its extensions).
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -83,3 +85,42 @@ extern "C" int mg_generate_f16(int32_t rows, int32_t cols, double sparsity, cons
     }
     return BCSR_OK;
 }
+
+extern "C" int mg_generate_bcsr(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,
+                                int32_t bw, int32_t band_halfwidth, double inblock_density, int32_t value_dist,
+                                double lo, double hi, uint32_t seed, int32_t *hdr, int64_t *nnzb, int32_t **idx,
+                                float **data) {
+    if (!hdr || !nnzb || !idx || !data || rows < 0 || cols < 0) return BCSR_ERR_INVALID_ARG;
+    if (bh <= 0 || bw <= 0) return BCSR_ERR_BLOCK_SIZE;
+    if (rows % bh || cols % bw) return BCSR_ERR_SHAPE;
+    *idx = nullptr;
+    *data = nullptr;
+    mg::GenSpec spec;
+    int rc = make_spec(sparsity, pattern, bh, bw, band_halfwidth, inblock_density, value_dist, lo, hi, spec);
+    if (rc != BCSR_OK) return rc;
+    mg::BcsrArrays a;
+    try {
+        mg::generate_bcsr(rows, cols, spec, seed, bh, bw, a);
+    } catch (const std::invalid_argument &) {
+        return BCSR_ERR_UNKNOWN_PATTERN;
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
+    const size_t n = a.idx.size();
+    std::copy(a.hdr.begin(), a.hdr.end(), hdr);
+    *nnzb = static_cast<int64_t>(n);
+    *idx = static_cast<int32_t *>(std::malloc(sizeof(int32_t) * (n ? n : 1)));
+    *data = static_cast<float *>(std::malloc(sizeof(float) * (a.data.size() ? a.data.size() : 1)));
+    if (!*idx || !*data) {
+        std::free(*idx);
+        std::free(*data);
+        *idx = nullptr;
+        *data = nullptr;
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
+    std::copy(a.idx.begin(), a.idx.end(), *idx);
+    std::copy(a.data.begin(), a.data.end(), *data);
+    return BCSR_OK;
+}
+
+extern "C" void mg_free(void *p) { std::free(p); }

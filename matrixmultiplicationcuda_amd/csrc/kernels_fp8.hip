@@ -8,12 +8,16 @@
 // bytes), so that 16 bytes are exactly one lane's half of an MFMA B operand
 // and no transpose is left for the product kernel.
 //
-// The product kernel walks quads of A blocks.  In one 16x16x128 MFMA each
-// 32-wide K group (lane group g of the wave) holds one block in its first 16
-// k and zeros in the second, so the MFMA's per-lane E8M0 operand scales are
-// exactly the per-block A scale and the per-slab B scale: the scaling costs no
-// VALU work.  Half of each MFMA's K is padding; the MFMA pipe is not the
-// bound (fp8 runs at twice the fp16 rate).
+// The product kernel walks quads of A blocks.  The scaled 16x16x128 MFMA
+// takes one E8M0 scale per row (A) or column (B) for each of four 32-wide K
+// groups; K group t is read from lane group t's scale operand and covers bytes
+// 16 (t >> 1) .. + 15 of lane groups 2 (t & 1) and 2 (t & 1) + 1.  Quad slot t
+// therefore puts its block in those bytes of lane group 2 (t & 1) and zeros in
+// lane group 2 (t & 1) + 1, so the per-block A scale and the per-slab B scale
+// are applied by the MFMA itself at no VALU cost.  Half of every operand is
+// padding: lane groups 1 and 3 read through an out-of-range buffer offset,
+// which returns zeros without memory traffic.  The MFMA pipe is not the bound
+// (fp8 runs at twice the fp16 rate).
 #include <algorithm>
 #include <cstdio>
 
@@ -92,27 +96,54 @@ __global__ __launch_bounds__(256) void bcsr_spmm_fp8(SpmmArgs a) {
         for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
-    const u32x4 *vals = a.vals + lane;
+    const bool data_lane = (g & 1) == 0;       // lane groups 0 and 2 carry blocks
+    const bool full_tile = n_base + 16 * EPL <= a.N;
+    const int lane32 = (g >> 1) * 16 + j;      // index among the stored lanes
+    const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
     for (int q = q0; q < q1; ++q) {
-        const int c = a.grp_col[4 * q + g];
+        const int4 cq = reinterpret_cast<const int4 *>(a.grp_col)[q];
+        const int c_scale = g == 0 ? cq.x : (g == 1 ? cq.y : (g == 2 ? cq.z : cq.w));
+        const int c_lo = (g >> 1) ? cq.y : cq.x;  // slots 2 (g >> 1) .. for halves 0 / 1
+        const int c_hi = (g >> 1) ? cq.w : cq.z;
         const int sa = static_cast<int>((a.ascale[q] >> (8 * g)) & 0xffu);
-        const int sb = a.b8scale[static_cast<size_t>(c) * a.b_chunks + chunk];
-        const unsigned off = static_cast<unsigned>(c) * (16u * N) + lane_off;
-        u32x4 bq[EPL];
+        const int sb = a.b8scale[static_cast<size_t>(c_scale) * a.b_chunks + chunk];
+        const unsigned base_lo = static_cast<unsigned>(c_lo) * (16u * N) + lane_off;
+        const unsigned base_hi = static_cast<unsigned>(c_hi) * (16u * N) + lane_off;
+        u32x4 blo[EPL], bhi[EPL];
+        if (full_tile) {
+            // every column of the tile exists: step columns through soffset
+            const unsigned off_lo = data_lane ? base_lo : kOutOfRange;
+            const unsigned off_hi = data_lane ? base_hi : kOutOfRange;
 #pragma unroll
-        for (int e = 0; e < EPL; ++e)
-            bq[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off + 256u * e), 0, 0);
+            for (int e = 0; e < EPL; ++e) {
+                blo[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off_lo), 256 * e, 0);
+                bhi[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off_hi), 256 * e, 0);
+            }
+        } else {
+            // ragged last tile: the range check covers voffset only, so every
+            // column offset goes there and missing columns read as zeros
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const bool ok = data_lane && n_base + 16 * e + j < a.N;
+                const unsigned ol = ok ? base_lo + 256u * e : kOutOfRange;
+                const unsigned oh = ok ? base_hi + 256u * e : kOutOfRange;
+                blo[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ol), 0, 0);
+                bhi[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(oh), 0, 0);
+            }
+        }
         i32x8 av[RS];
 #pragma unroll
         for (int s = 0; s < RS; ++s) {
-            const u32x4 x = vals[(static_cast<size_t>(q) * RS + s) * 64];
-            av[s] = i32x8{static_cast<int>(x.x), static_cast<int>(x.y), static_cast<int>(x.z), static_cast<int>(x.w),
-                          0, 0, 0, 0};
+            const unsigned aoff = data_lane ? static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 32 + lane32) * 32)
+                                            : kOutOfRange;
+            const u32x4 x0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
+            const u32x4 x1 = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
+            av[s] = __builtin_bit_cast(i32x8, (u32x8{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w}));
         }
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
-            const i32x8 bv = i32x8{static_cast<int>(bq[e].x), static_cast<int>(bq[e].y), static_cast<int>(bq[e].z),
-                                   static_cast<int>(bq[e].w), 0, 0, 0, 0};
+            const i32x8 bv = __builtin_bit_cast(
+                i32x8, (u32x8{blo[e].x, blo[e].y, blo[e].z, blo[e].w, bhi[e].x, bhi[e].y, bhi[e].z, bhi[e].w}));
 #pragma unroll
             for (int s = 0; s < RS; ++s)
                 acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av[s], bv, acc[s][e], 0, 0, 0, sa, 0, sb);
@@ -202,6 +233,7 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     a.map_mode = ls.map_mode;
     a.b_bytes = static_cast<int>(q_bytes);
     a.b_chunks = chunks;
+    a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 1024, INT32_MAX));
     a.wpb = ls.wpb;
     hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
     return hip_status(hipGetLastError());

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void bcsr_spmm_pair(SpmmArgs a) {
         auto issue = [&](int p, int2 cc, BRow<EPL>(&raw)[8], AOps<RS, PREC> &av) {
             const int pc = p < p1 ? p : p1 - 1;
             const unsigned off =
-                p < p1 ? static_cast<unsigned>(sel_hi ? cc.y : cc.x) * slab_bytes + lane_off : 0xfffffff0u;
+                p < p1 ? static_cast<unsigned>(sel_hi ? cc.y : cc.x) * slab_bytes + lane_off : kOutOfRange;
             load_patch<EPL>(rsrc, off, row_bytes, raw);
 #pragma unroll
             for (int s = 0; s < RS; ++s)

@@ -153,6 +153,54 @@ void generate_half_into(half *out, int rows, int cols, const GenSpec &spec, unsi
     });
 }
 
+void generate_bcsr(int rows, int cols, const GenSpec &spec, unsigned int seed, int bh, int bw, BcsrArrays &out) {
+    const std::string &p = spec.pattern;
+    bool aligned = false;
+    if (p == "blockrandom" || p == "blockrandom_rho" || p == "blockband") aligned = spec.bh == bh && spec.bw == bw;
+    if (p == "pattern32by16") aligned = bh == 32 && bw == 16;
+    if (p == "pattern64by16") aligned = bh == 64 && bw == 16;
+    if (!aligned) throw std::invalid_argument("generate_bcsr: pattern " + p + " does not generate " + std::to_string(bh) +
+                                              "x" + std::to_string(bw) + " blocks");
+    if (bh <= 0 || bw <= 0 || rows % bh || cols % bw)
+        throw std::invalid_argument("generate_bcsr: matrix is not a whole number of blocks");
+    const int nbr = rows / bh;
+    const size_t blk = static_cast<size_t>(bh) * bw;
+    out.hdr.assign(nbr + 1, 0);
+    out.idx.clear();
+    out.data.clear();
+    // Blocks are drawn one after the other in (block-row, block-column) order,
+    // so values arrive block-contiguously: collect one block, then keep it if
+    // any value is nonzero as a half.
+    int cur_bi = -1, cur_bj = -1;
+    std::vector<float> cur(blk, 0.0f);
+    auto flush = [&]() {
+        if (cur_bi < 0) return;
+        bool nz = false;
+        for (float v : cur)
+            if (float2half_rn(v) != static_cast<half>(0.0f)) {
+                nz = true;
+                break;
+            }
+        if (nz) {
+            out.idx.push_back(cur_bj);
+            out.data.insert(out.data.end(), cur.begin(), cur.end());
+            out.hdr[cur_bi + 1]++;
+        }
+        std::fill(cur.begin(), cur.end(), 0.0f);
+    };
+    generate_core<float>(rows, cols, spec, seed, [&](int i, int j, float v) {
+        const int bi = i / bh, bj = j / bw;
+        if (bi != cur_bi || bj != cur_bj) {
+            flush();
+            cur_bi = bi;
+            cur_bj = bj;
+        }
+        cur[static_cast<size_t>(i % bh) * bw + (j % bw)] = v;
+    });
+    flush();
+    for (int r = 0; r < nbr; ++r) out.hdr[r + 1] += out.hdr[r];
+}
+
 template Matrix<float> generate_matrix<float>(int, int, double, const std::string &, int, unsigned int);
 template Matrix<double> generate_matrix<double>(int, int, double, const std::string &, int, unsigned int);
 template Matrix<int> generate_matrix<int>(int, int, double, const std::string &, int, unsigned int);

@@ -4,7 +4,11 @@ These modes have no reference counterpart (the reference is fp16 only).  Each
 is checked twice:
   1. against the exact product of the values the mode stores (the oracle's
      restatement of the quantisers, dequantised, multiplied in fp64): the
-     kernel may differ only by fp32 accumulation, bound 1e-4 * (|Aq|.|Bq|)_ij;
+     16-bit modes may differ only by fp32 accumulation, bound
+     1e-4 * (|Aq|.|Bq|)_ij.  FP8 gets 2^-11: the scaled 16x16x128 FP8 MFMA
+     does not add its products in IEEE fp32 (on MI355X its result differs
+     from the exact sum of the exact products by up to 1.2e-4 of their
+     magnitude sum; no fixed-alignment truncation model reproduces it);
   2. against the FP64 product of the unquantised A (fp32 generator values)
      and B (the fp16 input), with each mode's error bound per entry relative
      to (|A|.|B|)_ij: fp16 2^-11, bf16 2^-8 + 2^-8 (A and the B operand are
@@ -62,7 +66,9 @@ def test_precision_mode(mode, bh, N):
     Bq = oracle.quantized_B(mode, B16.view(np.uint16))
     Cq, Cqabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, Aq32, bh, 16, Bq, threads=8)
     err_q = np.abs(C - Cq)
-    assert np.all(err_q <= 1e-4 * Cqabs + 1e-30), f"{mode}: kernel differs from its stored values"
+    tight = 2.0**-11 if mode == "fp8" else 1e-4
+    log(f"  {mode}: max |C - Cq| / (|Aq||Bq|) {np.max(err_q / np.maximum(Cqabs, 1e-30)):.3e}")
+    assert np.all(err_q <= tight * Cqabs + 1e-30), f"{mode}: kernel differs from its stored values"
     Bf = B16.astype(np.float32)
     C64, Cabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, data32, bh, 16, Bf, threads=8)
     nz = Cabs > 0

@@ -99,6 +99,34 @@ def test_bcsr_equals_oracle(bh, bw):
     assert np.array_equal(h.data.view(np.uint16), data)
 
 
+@pytest.mark.parametrize("pattern,bh,gbh,kw", [
+    ("blockrandom", 16, 16, {}), ("blockrandom", 32, 32, {"value_dist": "loguniform"}),
+    ("blockrandom_rho", 16, 16, {"inblock_density": 0.3}), ("blockband", 16, 16, {"band_halfwidth": 2}),
+    ("pattern32by16", 32, 16, {}), ("pattern64by16", 64, 16, {})])
+def test_generate_bcsr_equals_dense_path(pattern, bh, gbh, kw):
+    """The block-by-block generator == the oracle's dense generator followed by
+    the oracle's BCSR builder (same blocks, same fp32 values)."""
+    bw = 16 if pattern.startswith("pattern") else bh
+    h = mm.generate_bcsr(256, 320, 0.8, pattern, bh=bh, bw=bw, seed=17, **kw)
+    A = oracle.generate_ex(256, 320, 0.8, pattern, bh=gbh, bw=gbh if bw == gbh else bw, seed=17,
+                           **kw)
+    hdr, idx, _ = oracle.bcsr_from_dense(oracle.f32_to_f16_bits(A), bh, bw)
+    assert np.array_equal(h.hdr, hdr) and np.array_equal(h.idx, idx)
+    assert np.array_equal(h.data, oracle.bcsr_values_f32(A, hdr, idx, bh, bw))
+
+
+def test_generate_bcsr_rejects_unaligned():
+    with pytest.raises(mm.BcsrError) as e:
+        mm.generate_bcsr(64, 64, 0.5, "random", bh=16, bw=16)
+    assert e.value.status == -8
+    with pytest.raises(mm.BcsrError) as e:
+        mm.generate_bcsr(64, 64, 0.5, "pattern64by16", bh=32, bw=16)  # its blocks are 64 x 16
+    assert e.value.status == -8
+    with pytest.raises(mm.BcsrError) as e:
+        mm.generate_bcsr(72, 64, 0.5, "blockrandom", bh=16, bw=16)
+    assert e.value.status == -3
+
+
 @pytest.mark.parametrize("bh,bw,rows,cols,code", [(8, 16, 64, 64, -2), (16, 24, 64, 64, -2),
                                                   (16, 16, 40, 64, -3), (32, 16, 48, 64, -3)])
 def test_bcsr_rejects_bad_shapes(bh, bw, rows, cols, code):
