@@ -42,7 +42,7 @@ enum bcsr_status {
 enum bcsr_precision {
     BCSR_PREC_F16 = 0,
     BCSR_PREC_BF16 = 1,
-    BCSR_PREC_FP8_E4M3 = 2, /* per-block fp32 scale */
+    BCSR_PREC_FP8_E4M3 = 2, /* per-block power-of-two (E8M0) scale; B quantised per call */
     BCSR_PREC_F16_HILO = 3  /* hi + lo fp16 split, near-fp32 accuracy */
 };
 

@@ -138,8 +138,12 @@ void orc_bcsr_spmm_f64(const int32_t *hdr, const int32_t *idx, const float *data
  * storage mode.  Round to nearest even, finite overflow saturates to +-448.
  * Scales are powers of two stored as E8M0 bytes (2^(e - 127)): the smallest
  * 2^k with absmax / 2^k <= 448, 127 for an all-zero group.
- * pinned: exhaustive decode table vs the OCP definition, round trip of all
- * 254 finite codes, midpoints (ties to even), saturation; see tests. */
+ * pinned (tests/test_oracle_precision.py): all 256 codes decode as PyTorch's
+ * float8_e4m3fn; encoding equals PyTorch's RNE cast on a 200k-value sweep
+ * plus every tie between neighbouring codes and the subnormal boundaries;
+ * saturation, -0 and NaN as special cases; E8M0 choice against its defining
+ * inequality; group and B-slab quantisers against PyTorch's cast of each
+ * group divided by its own scale. */
 uint8_t orc_f2e4m3(float f);
 float orc_e4m32f(uint8_t b);
 uint8_t orc_e8m0_for_absmax(float absmax);
@@ -151,9 +155,10 @@ void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale);
 
 /* ---- other storage modes (extensions) ---------------------------------
  * bf16: float -> bfloat16 round to nearest even.  hi+lo: hi = fp16(a),
- * lo = fp16((a - hi) * 2^11), a ~ hi + 2^-11 lo.  pinned: bf16 vs numpy's
- * float32 bit manipulation identity on all halves; hi+lo by the exactness of
- * a - hi (checked in tests). */
+ * lo = fp16((a - hi) * 2^11), a ~ hi + 2^-11 lo.  pinned
+ * (tests/test_oracle_precision.py): bf16 equals PyTorch's bfloat16 cast on a
+ * wide sweep; hi equals numpy's float16 cast, lo numpy's cast of the exact
+ * residual, and hi + 2^-11 lo reconstructs a to 2^-21 relative. */
 uint16_t orc_f2bf16(float f);
 float orc_bf162f(uint16_t b);
 void orc_hilo_split(float a, uint16_t *hi, uint16_t *lo);

@@ -1,0 +1,148 @@
+"""Pins of the oracle's storage-precision functions (bf16, FP8 E4M3, E8M0
+scales, hi+lo fp16).  These modes have no reference counterpart; each oracle
+function is checked against an independent implementation (PyTorch's CPU
+float8_e4m3fn / bfloat16 casts, numpy float16) or against the closed form the
+format fixes, so that a wrong bias, rounding mode, scale direction or slab
+index fails here before it is used to judge the kernels.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+torch = pytest.importorskip("torch")
+
+
+def _sweep_f32(n=200_000, seed=0):
+    """Random floats over many binades, plus the E4M3 rounding ties and
+    subnormal boundaries."""
+    rng = np.random.default_rng(seed)
+    mag = 2.0 ** rng.uniform(-14, 9.5, n)
+    x = (mag * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+    codes = np.arange(256, dtype=np.uint8)
+    vals = oracle.e4m3_to_f32(codes)
+    vals = vals[np.isfinite(vals)].astype(np.float64)
+    vals = np.unique(vals)
+    mids = (vals[:-1] + vals[1:]) / 2.0  # exact ties between neighbours
+    extra = np.concatenate([vals, mids, [2.0**-10, 3 * 2.0**-10, 2.0**-6, 2.0**-6 - 2.0**-10]])
+    return np.concatenate([x, extra.astype(np.float32), -extra.astype(np.float32)])
+
+
+def test_e4m3_decode_all_codes():
+    codes = np.arange(256, dtype=np.uint8)
+    ours = oracle.e4m3_to_f32(codes)
+    theirs = torch.from_numpy(codes.copy()).view(torch.float8_e4m3fn).float().numpy()
+    nan = np.isnan(theirs)
+    assert np.array_equal(np.isnan(ours), nan) and nan.sum() == 2  # 0x7f, 0xff
+    assert np.array_equal(ours[~nan], theirs[~nan])
+
+
+def test_e4m3_encode_matches_torch_rne():
+    x = _sweep_f32()
+    x = x[np.abs(x) < 464.0]  # torch turns 480 and above into NaN; saturation below
+    ours = oracle.f32_to_e4m3(x)
+    theirs = torch.from_numpy(x).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(ours, theirs)
+
+
+def test_e4m3_saturation_and_specials():
+    big = np.array([464.0, 480.0, 1e4, np.inf, -500.0, -np.inf], dtype=np.float32)
+    assert list(oracle.f32_to_e4m3(big)) == [0x7E, 0x7E, 0x7E, 0x7E, 0xFE, 0xFE]
+    assert oracle.f32_to_e4m3(np.array([np.nan], np.float32))[0] & 0x7F == 0x7F
+    assert oracle.f32_to_e4m3(np.array([-0.0], np.float32))[0] == 0x80
+
+
+@pytest.mark.parametrize("absmax", [0.0, 1e-30, 2.0**-130, 1e-3, 0.5, 1.0, 447.9, 448.0, 448.01,
+                                    896.0, 1e30, 3.4e38])
+def test_e8m0_is_smallest_power_of_two(absmax):
+    """scale 2^k with absmax / 2^k <= 448 and k minimal (k >= -127; 127 for 0)."""
+    _, sc = oracle.quant_e4m3_groups(np.array([absmax], np.float32), 1)
+    k = int(sc[0]) - 127
+    if absmax == 0.0:
+        assert k == 0
+        return
+    a = float(np.float32(absmax))
+    assert a / 2.0**k <= 448.0
+    if k > -127:
+        assert a / 2.0 ** (k - 1) > 448.0
+
+
+def test_group_quantiser_uses_each_groups_scale():
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((6, 256)) * np.array([1e-3, 1.0, 7.0, 300.0, 5e3, 0.0])[:, None])
+    x = x.astype(np.float32)
+    q, sc = oracle.quant_e4m3_groups(x, 256)
+    for g in range(6):
+        s = 2.0 ** (int(sc[g]) - 127)
+        want = torch.from_numpy((x[g] / np.float32(s)).astype(np.float32)).to(
+            torch.float8_e4m3fn).view(torch.uint8).numpy()
+        assert np.array_equal(q[g], want)
+        # dequantised values are within half an E4M3 step of the input
+        deq = oracle.e4m3_to_f32(q[g]).astype(np.float64) * s
+        amax = np.abs(x[g]).max()
+        if amax > 0:
+            tol = np.maximum(np.abs(x[g]) * 2.0**-4, 2.0**-10 * s)
+            assert np.all(np.abs(deq - x[g]) <= tol)
+
+
+def test_b_quantiser_slabs_and_chunks():
+    """One scale per (16-row slab, 256-column chunk); the last chunk ragged."""
+    rng = np.random.default_rng(5)
+    K, N = 48, 600
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    B[16:32, 256:512] *= 1e-3
+    B[32:48, 512:] *= 50.0
+    q, sc = oracle.quant_e4m3_B(B)
+    assert sc.shape == (3, 3)
+    for s in range(3):
+        for c in range(3):
+            blk = B[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)]
+            k = int(sc[s, c]) - 127
+            amax = float(np.abs(blk).max())
+            assert amax / 2.0**k <= 448.0 < amax / 2.0 ** (k - 1)
+            want = torch.from_numpy((blk / np.float32(2.0**k)).astype(np.float32)).to(
+                torch.float8_e4m3fn).view(torch.uint8).numpy()
+            assert np.array_equal(q[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)], want)
+
+
+def test_bf16_matches_torch():
+    x = _sweep_f32(seed=7)
+    x = np.concatenate([x, np.array([3.3895314e38, 1e-40, -1e-45], np.float32)])
+    ours = oracle.f32_to_bf16_bits(x)
+    theirs = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(ours, theirs)
+    assert np.array_equal(oracle.bf16_bits_to_f32(ours), torch.from_numpy(x).to(torch.bfloat16)
+                          .float().numpy())
+
+
+def test_hilo_split_reconstructs_to_fp32_level():
+    """hi = fp16(a); lo = fp16((a - hi) * 2^11); hi + lo * 2^-11 == a up to
+    the rounding of lo (2^-11 relative to a - hi, itself <= 2^-11 |a|)."""
+    rng = np.random.default_rng(11)
+    a = (10.0 ** rng.uniform(-3, 2, 50_000) * rng.choice([-1, 1], 50_000)).astype(np.float32)
+    hi, lo = oracle.hilo_split(a)
+    assert np.array_equal(hi, a.astype(np.float16).view(np.uint16))
+    hif = oracle.f16_bits_to_f32(hi).astype(np.float64)
+    lof = oracle.f16_bits_to_f32(lo).astype(np.float64)
+    r = ((a.astype(np.float64) - hif) * 2048.0).astype(np.float32).astype(np.float16)
+    assert np.array_equal(lo, r.view(np.uint16))
+    err = np.abs(hif + lof * 2.0**-11 - a)
+    assert np.all(err <= np.abs(a) * 2.0**-21 + 2.0**-35)
+    assert math.isclose(float(np.median(err / np.abs(a))), 0.0, abs_tol=2e-7)
+
+
+def test_planted_precision_mistakes_are_caught():
+    """A wrong bias or a truncating rounding in the E4M3 encoder would differ
+    from torch on the sweep (checked on a copy of the oracle's decode)."""
+    x = _sweep_f32(20_000, seed=9)
+    x = x[np.abs(x) < 464.0]
+    good = oracle.f32_to_e4m3(x)
+    theirs = torch.from_numpy(x).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    assert np.array_equal(good, theirs)
+    truncated = np.array([oracle.f32_to_e4m3(np.array([np.float32(v) * np.float32(1 - 2**-5)],
+                                                      np.float32))[0] for v in x[:2000]])
+    assert not np.array_equal(truncated, theirs[:2000])
+    shifted = oracle.f32_to_e4m3((x * 2).astype(np.float32))  # a bias off by one
+    assert not np.array_equal(shifted, theirs)
