@@ -103,6 +103,39 @@ int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *sc
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
                           char *buf, size_t buflen);
 
+/* ---- batches: many products in one persistent launch -------------------------
+ * No reference counterpart (the reference launches one kernel per product).
+ * bcsr_batch_create binds count products C_i = plans[i] . B_i (N[i] columns;
+ * B_i, C_i device pointers as for bcsr_spmm) and builds the device work queue;
+ * all plans share one precision and the current device.  tile_n[i] is the
+ * output tile width of problem i (0 or tile_n == NULL: automatic); widths the
+ * persistent kernel is not compiled for give BCSR_ERR_UNSUPPORTED
+ * (bcsr_batch_tile_supported tells).  bcsr_batch_run enqueues the batch
+ * (counter reset + persistent kernel; FP8 first quantises every B);
+ * bcsr_batch_run_graph captures that sequence into a hipGraph on its first
+ * call and replays it (stream must not be NULL).  Results equal the
+ * per-matrix launches bit for bit. */
+typedef struct bcsr_batch bcsr_batch;
+typedef struct {
+    int32_t waves_per_cu; /* persistent waves per CU: 0 = 8 */
+    int32_t bin_work;     /* work per queue item, in 16 x 16 x 32 MFMA-tile steps: 0 = auto */
+    int32_t reserved[6];
+} bcsr_batch_cfg;
+typedef struct {
+    int32_t count, items, grid, precision;
+    double flops;         /* sum over problems of 2 * nnzb * bh * bw * N */
+    int64_t device_bytes; /* queue, descriptors and FP8 workspaces */
+} bcsr_batch_info;
+int bcsr_batch_tile_supported(int32_t precision, int32_t dev_bh, int32_t tile_n);
+int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C,
+                      const int32_t *N, const int32_t *tile_n, int32_t count,
+                      const bcsr_batch_cfg *cfg, bcsr_batch **out);
+int bcsr_batch_run(bcsr_batch *batch, void *stream);
+int bcsr_batch_run_graph(bcsr_batch *batch, void *stream);
+int bcsr_batch_get_info(const bcsr_batch *batch, bcsr_batch_info *info);
+int bcsr_batch_tiles(const bcsr_batch *batch, int32_t *tiles);
+int bcsr_batch_destroy(bcsr_batch *batch);
+
 /* ---- drop-in device arrays -------------------------------------------------
  * Replaces BCSRMatrix::copyToDevice (formats/BCSRMatrix.cu:106-132): builds an
  * fp16 plan from host arrays and returns its device arrays.  *gpu_hdr is the

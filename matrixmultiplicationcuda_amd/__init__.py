@@ -88,6 +88,17 @@ class bcsr_plan_info(ctypes.Structure):
                 ("dev_pairs", ctypes.c_int64), ("device_bytes", ctypes.c_int64)]
 
 
+class bcsr_batch_cfg(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("bin_work", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
+class bcsr_batch_info(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int32), ("items", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("flops", ctypes.c_double),
+                ("device_bytes", ctypes.c_int64)]
+
+
 _lib_handle = None
 
 _P = ctypes.c_void_p
@@ -114,6 +125,14 @@ def _declare(lib):
     f("bcsr_spmm_repeat", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _I32, _P)
     f("bcsr_spmm_kernel_name", i, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), ctypes.c_char_p,
       ctypes.c_size_t)
+    f("bcsr_batch_tile_supported", i, _I32, _I32, _I32)
+    f("bcsr_batch_create", i, _P, _P, _P, _P, _P, _I32, ctypes.POINTER(bcsr_batch_cfg),
+      ctypes.POINTER(_P))
+    f("bcsr_batch_run", i, _P, _P)
+    f("bcsr_batch_run_graph", i, _P, _P)
+    f("bcsr_batch_get_info", i, _P, ctypes.POINTER(bcsr_batch_info))
+    f("bcsr_batch_tiles", i, _P, _P)
+    f("bcsr_batch_destroy", i, _P)
     f("bcsr_copy_to_device", i, _P, _P, _P, _I32, _I32, _I32, _I32, ctypes.POINTER(_P),
       ctypes.POINTER(_P), ctypes.POINTER(_P))
     f("bcsr_release_device", i, _P)
@@ -484,6 +503,62 @@ class Plan:
     def close(self):
         if self._handle:
             check(self._lib.bcsr_plan_destroy(self._handle), "bcsr_plan_destroy")
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many products C_i = plans[i] . B_i in one persistent launch.
+
+    plans: Plan objects of one precision; Bs / Cs: DeviceArrays (K_i x N_i
+    fp16, M_i x N_i fp32); tiles: per-problem tile widths (None/0 = auto)."""
+
+    def __init__(self, plans, Bs, Cs, tiles=None, waves_per_cu=0, bin_work=0):
+        self._lib = lib()
+        n = len(plans)
+        if len(Bs) != n or len(Cs) != n:
+            raise ValueError("plans, Bs and Cs must have the same length")
+        self._keep = (list(plans), list(Bs), list(Cs))
+        hp = (_P * max(n, 1))(*[p._handle.value for p in plans])
+        hb = (_P * max(n, 1))(*[b.ptr for b in Bs])
+        hc = (_P * max(n, 1))(*[c.ptr for c in Cs])
+        nn = np.array([b.shape[1] for b in Bs] or [0], dtype=np.int32)
+        tt = np.array(list(tiles) if tiles is not None else [0] * n, dtype=np.int32)
+        if tt.size == 0:
+            tt = np.zeros(1, np.int32)
+        cfg = bcsr_batch_cfg(waves_per_cu, bin_work)
+        self._handle = ctypes.c_void_p()
+        check(self._lib.bcsr_batch_create(hp, hb, hc, _ptr(nn), _ptr(tt), n, ctypes.byref(cfg),
+                                          ctypes.byref(self._handle)), "bcsr_batch_create")
+        self.count = n
+
+    @property
+    def info(self) -> bcsr_batch_info:
+        inf = bcsr_batch_info()
+        check(self._lib.bcsr_batch_get_info(self._handle, ctypes.byref(inf)), "batch info")
+        return inf
+
+    @property
+    def tiles(self):
+        out = np.zeros(max(self.count, 1), np.int32)
+        check(self._lib.bcsr_batch_tiles(self._handle, _ptr(out)), "batch tiles")
+        return out[:self.count]
+
+    def run(self, stream=None, graph=False):
+        if graph:
+            check(self._lib.bcsr_batch_run_graph(self._handle, stream_handle(stream)),
+                  "bcsr_batch_run_graph")
+        else:
+            check(self._lib.bcsr_batch_run(self._handle, stream_handle(stream)), "bcsr_batch_run")
+
+    def close(self):
+        if self._handle:
+            check(self._lib.bcsr_batch_destroy(self._handle), "bcsr_batch_destroy")
             self._handle = ctypes.c_void_p()
 
     def __del__(self):

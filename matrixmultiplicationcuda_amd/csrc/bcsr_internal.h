@@ -29,6 +29,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "bcsr_mi355x.h"
 
@@ -59,6 +60,8 @@ struct bcsr_plan {
     std::mutex *ws_mu = nullptr;
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
+    // host copies of grp_ptr and row_order (work-item construction)
+    std::vector<int32_t> h_grp_ptr, h_row_order;
 };
 
 // Launch geometry resolved from a bcsr_launch_cfg.

@@ -276,6 +276,8 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     plan->rs = rs;
     plan->dev_rows = dev_rows;
     plan->group = G;
+    plan->h_grp_ptr = std::move(grp_ptr);
+    plan->h_row_order = std::move(row_order);
     if (fp8) plan->ws_mu = new std::mutex();
     bcsr_plan_info &inf = plan->info;
     inf.rows = block_rows * bh;

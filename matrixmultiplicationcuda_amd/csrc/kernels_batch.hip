@@ -1,0 +1,422 @@
+// kernels_batch.hip -- many independent BCSR x dense products in one
+// persistent launch (no reference counterpart: the reference launches one
+// kernel per product).
+//
+// A batch binds count products C_i = A_i . B_i (plans of one precision, each
+// with its own block shape, N and output tile width).  Their work is cut into
+// items (problem, column tile, run of device block-rows taken from the
+// problem's longest-first row order), sized by a work target and sorted by
+// decreasing estimated work.  A persistent grid of waves pulls items from a
+// device counter (one atomic per item) until the queue is empty, and runs each
+// with the per-matrix kernels' own work functions, so the batched results are
+// bit-identical to the per-matrix launches.  A run is two kernels: a one-thread
+// reset of the counter and the persistent kernel (plus, for FP8, one batched
+// B-quantisation kernel first); the sequence can be captured once into a
+// hipGraph and replayed.  Memory is never cleared with hipMemsetAsync, so the
+// captured graph holds kernel nodes only.
+#include <algorithm>
+#include <cstdio>
+#include <numeric>
+
+#include "fp8_device.h"
+#include "pair_device.h"
+
+namespace {
+
+enum { BPREC_F16 = 0, BPREC_BF16 = 1, BPREC_FP8 = 2, BPREC_HILO = 3 };
+
+struct BatchMat {
+    SpmmArgs a;
+    int key;  // rs * 16 + epl
+};
+
+struct QuantJob {
+    const _Float16 *B;
+    uint8_t *B8;
+    uint8_t *scale;
+    int N;
+    int chunks;
+    int wg0;  // first workgroup index of this job (prefix over slabs * chunks)
+};
+
+// Block shapes the persistent kernel is compiled for: the kernel's register
+// allocation is the maximum over its branches, so the large ones (64-row
+// blocks with 128-column tiles, hi+lo beyond 8 accumulator tiles) are left to
+// the per-matrix kernels.
+__host__ __device__ constexpr bool batch_shape(int prec, int rs, int epl) {
+    const int h = prec == BPREC_HILO ? 2 : 1;
+    return (rs == 1 || rs == 2 || rs == 4) && (epl == 2 || epl == 4 || epl == 8) && rs * epl * h <= 16;
+}
+
+template <int PREC, int RS, int EPL>
+__device__ __forceinline__ void run_rows(const SpmmArgs &a, int tile, int s0, int s1) {
+    if constexpr (batch_shape(PREC, RS, EPL)) {
+        for (int s = s0; s < s1; ++s) {
+            const int br = a.row_order[s];
+            if constexpr (PREC == BPREC_FP8)
+                fp8dev::fp8_work<RS, EPL>(a, br, tile);
+            else
+                pairdev::pair_work<RS, EPL, true, PREC>(a, br, tile);
+        }
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void run_item(const SpmmArgs &a, int key, int tile, int s0, int s1) {
+    switch (key) {
+        case 0x12: run_rows<PREC, 1, 2>(a, tile, s0, s1); break;
+        case 0x14: run_rows<PREC, 1, 4>(a, tile, s0, s1); break;
+        case 0x18: run_rows<PREC, 1, 8>(a, tile, s0, s1); break;
+        case 0x22: run_rows<PREC, 2, 2>(a, tile, s0, s1); break;
+        case 0x24: run_rows<PREC, 2, 4>(a, tile, s0, s1); break;
+        case 0x28: run_rows<PREC, 2, 8>(a, tile, s0, s1); break;
+        case 0x42: run_rows<PREC, 4, 2>(a, tile, s0, s1); break;
+        case 0x44: run_rows<PREC, 4, 4>(a, tile, s0, s1); break;
+        default: break;
+    }
+}
+
+__global__ void batch_reset(int *counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+// One wave per workgroup; every wave leaves once the counter passes n_items.
+template <int PREC>
+__global__ __launch_bounds__(64) void bcsr_batch_persistent(const BatchMat *__restrict__ mats,
+                                                            const int4 *__restrict__ items, int n_items,
+                                                            int *counter) {
+    for (;;) {
+        int it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(counter, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+        if (it >= n_items) return;
+        const int4 w = items[it];
+        const BatchMat &m = mats[w.x];
+        const SpmmArgs a = m.a;
+        run_item<PREC>(a, m.key, w.y, w.z, w.w);
+    }
+}
+
+// FP8: every problem's B quantised in one launch, workgroups striding over
+// the (job, slab, chunk) list.
+__global__ __launch_bounds__(256) void batch_quant(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
+    for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
+        int lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].wg0 <= w) lo = mid; else hi = mid - 1;
+        }
+        const QuantJob &J = jobs[lo];
+        const int local = w - J.wg0;
+        fp8dev::quant_slab(J.B, J.B8, J.scale, J.N, J.chunks, local / J.chunks, local % J.chunks);
+        __syncthreads();  // the slab reduction's shared memory is reused next iteration
+    }
+}
+
+typedef void (*PersistentKernel)(const BatchMat *, const int4 *, int, int *);
+
+PersistentKernel pick_persistent(int prec) {
+    switch (prec) {
+        case BPREC_F16: return bcsr_batch_persistent<BPREC_F16>;
+        case BPREC_BF16: return bcsr_batch_persistent<BPREC_BF16>;
+        case BPREC_FP8: return bcsr_batch_persistent<BPREC_FP8>;
+        case BPREC_HILO: return bcsr_batch_persistent<BPREC_HILO>;
+        default: return nullptr;
+    }
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+struct bcsr_batch {
+    int device = 0;
+    int precision = 0;
+    int count = 0;
+    int n_items = 0;
+    int grid = 0;
+    int n_jobs = 0;
+    int quant_wg = 0;
+    int quant_grid = 0;
+    double flops = 0.0;
+    size_t bytes = 0;
+    void *d_base = nullptr;
+    BatchMat *d_mats = nullptr;
+    int4 *d_items = nullptr;
+    int *d_counter = nullptr;
+    QuantJob *d_jobs = nullptr;
+    std::vector<int32_t> tiles;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
+    if (b->n_jobs) {
+        hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(256), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
+        int rc = hip_status(hipGetLastError());
+        if (rc != BCSR_OK) return rc;
+    }
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, b->d_counter);
+    int rc = hip_status(hipGetLastError());
+    if (rc != BCSR_OK) return rc;
+    if (!b->n_items) return BCSR_OK;
+    PersistentKernel k = pick_persistent(b->precision);
+    hipLaunchKernelGGL(k, dim3(b->grid), dim3(64), 0, stream, b->d_mats, b->d_items, b->n_items, b->d_counter);
+    return hip_status(hipGetLastError());
+}
+
+int auto_batch_tile(const bcsr_plan *p, int N) {
+    int tile = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
+    while (tile > 32 && !batch_shape(p->info.precision, p->rs, tile / 16)) tile /= 2;
+    return tile;
+}
+
+}  // namespace
+
+extern "C" int bcsr_batch_tile_supported(int32_t precision, int32_t dev_bh, int32_t tile_n) {
+    if (dev_bh % 16 || tile_n % 16 || tile_n <= 0) return 0;
+    return batch_shape(precision, dev_bh / 16, tile_n / 16) ? 1 : 0;
+}
+
+extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C,
+                                 const int32_t *N, const int32_t *tile_n, int32_t count,
+                                 const bcsr_batch_cfg *cfg, bcsr_batch **out) {
+    if (!out || count < 0 || (count > 0 && (!plans || !B || !C || !N))) return BCSR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int dev = 0;
+    int rc = hip_status(hipGetDevice(&dev));
+    if (rc != BCSR_OK) return rc;
+    int n_cu = 0;
+    rc = hip_status(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (rc != BCSR_OK) return rc;
+    const int wpc = cfg && cfg->waves_per_cu > 0 ? cfg->waves_per_cu : 8;
+    if (wpc > 32) return BCSR_ERR_INVALID_ARG;
+    const int precision = count ? plans[0]->info.precision : BCSR_PREC_F16;
+    const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
+
+    std::vector<BatchMat> mats(count);
+    std::vector<int32_t> tiles(count);
+    std::vector<QuantJob> jobs;
+    std::vector<size_t> ws_off(count, 0);
+    size_t ws_total = 0;
+    int quant_wg = 0;
+    double flops = 0.0;
+    for (int i = 0; i < count; ++i) {
+        const bcsr_plan *p = plans[i];
+        if (!p) return BCSR_ERR_INVALID_ARG;
+        if (p->info.precision != precision) return BCSR_ERR_UNSUPPORTED;
+        if (p->device != dev) return BCSR_ERR_INVALID_ARG;
+        const int n = N[i];
+        if (n < 0) return BCSR_ERR_INVALID_ARG;
+        if (n % 16) return BCSR_ERR_SHAPE;
+        if (n && p->info.rows && !C[i]) return BCSR_ERR_INVALID_ARG;
+        if (n && p->info.cols && !B[i]) return BCSR_ERR_INVALID_ARG;
+        if ((reinterpret_cast<uintptr_t>(B[i]) & 15u) || (reinterpret_cast<uintptr_t>(C[i]) & 15u))
+            return BCSR_ERR_SHAPE;
+        const int64_t kn = static_cast<int64_t>(p->info.cols) * n;
+        if (kn * 2 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;  // 32-bit buffer offsets
+        if (fp8 && p->info.dev_pairs * p->rs * 1024 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
+        int t = tile_n && tile_n[i] > 0 ? tile_n[i] : auto_batch_tile(p, n);
+        if (t % 16 || !batch_shape(precision, p->rs, t / 16)) return BCSR_ERR_UNSUPPORTED;
+        tiles[i] = t;
+        flops += 2.0 * static_cast<double>(p->info.nnzb) * p->info.bh * p->info.bw * n;
+        SpmmArgs &a = mats[i].a;
+        a = SpmmArgs{};
+        a.grp_ptr = p->d_grp_ptr;
+        a.grp_col = p->d_grp_col;
+        a.vals = static_cast<const u32x4 *>(p->d_vals);
+        a.ascale = p->d_ascale;
+        a.row_order = p->d_row_order;
+        a.B = static_cast<const _Float16 *>(B[i]);
+        a.C = C[i];
+        a.N = n;
+        a.n_tiles = n ? (n + t - 1) / t : 0;
+        a.dev_rows = p->dev_rows;
+        a.b_bytes = static_cast<int>(kn * 2);
+        a.wpb = 1;
+        mats[i].key = p->rs * 16 + t / 16;
+        if (fp8 && n && p->info.cols) {
+            const int slabs = p->info.cols / 16, chunks = (n + 255) / 256;
+            ws_off[i] = ws_total;
+            ws_total += align256(static_cast<size_t>(kn)) + align256(static_cast<size_t>(slabs) * chunks);
+            a.b_bytes = static_cast<int>(kn);
+            a.b_chunks = chunks;
+            a.a_bytes = static_cast<int>(p->info.dev_pairs * p->rs * 1024);
+            QuantJob J{};
+            J.B = static_cast<const _Float16 *>(B[i]);
+            J.N = n;
+            J.chunks = chunks;
+            J.wg0 = quant_wg;
+            quant_wg += slabs * chunks;
+            jobs.push_back(J);
+        }
+    }
+
+    // work items: per problem and column tile, runs of rows (longest first)
+    // holding about `bin` units of work; a unit is one pair (or quad) step
+    // of a 16-column MFMA tile, and every row costs one more for its store
+    struct Item {
+        int4 w;
+        double work;
+    };
+    std::vector<Item> items;
+    double total_work = 0.0;
+    for (int i = 0; i < count; ++i) {
+        const bcsr_plan *p = plans[i];
+        const int nt = mats[i].a.n_tiles;
+        if (!nt) continue;
+        for (int r = 0; r < p->dev_rows; ++r)
+            total_work += (p->h_grp_ptr[r + 1] - p->h_grp_ptr[r] + 1.0) * p->rs * (tiles[i] / 16) * nt;
+    }
+    const int grid = n_cu * wpc;
+    // default: items of 64 units (fastest measured over the 96-problem
+    // batched workload on MI355X), fewer only when the batch is small
+    double bin = cfg && cfg->bin_work > 0 ? cfg->bin_work : std::clamp(total_work / (4.0 * grid), 8.0, 64.0);
+    for (int i = 0; i < count; ++i) {
+        const bcsr_plan *p = plans[i];
+        const int nt = mats[i].a.n_tiles;
+        if (!nt || !p->dev_rows) continue;
+        const double unit = p->rs * (tiles[i] / 16);
+        for (int t = 0; t < nt; ++t) {
+            int s0 = 0;
+            double acc = 0.0;
+            for (int s = 0; s < p->dev_rows; ++s) {
+                const int r = p->h_row_order[s];
+                acc += (p->h_grp_ptr[r + 1] - p->h_grp_ptr[r] + 1.0) * unit;
+                if (acc >= bin || s + 1 == p->dev_rows) {
+                    items.push_back({int4{i, t, s0, s + 1}, acc});
+                    s0 = s + 1;
+                    acc = 0.0;
+                }
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.work > y.work; });
+    if (items.size() > static_cast<size_t>(INT32_MAX)) return BCSR_ERR_UNSUPPORTED;
+
+    bcsr_batch *b = new (std::nothrow) bcsr_batch();
+    if (!b) return BCSR_ERR_OUT_OF_MEMORY;
+    const size_t off_m = 0;
+    const size_t off_i = align256(off_m + sizeof(BatchMat) * std::max(count, 1));
+    const size_t off_c = align256(off_i + sizeof(int4) * std::max<size_t>(items.size(), 1));
+    const size_t off_j = align256(off_c + sizeof(int));
+    const size_t off_w = align256(off_j + sizeof(QuantJob) * std::max<size_t>(jobs.size(), 1));
+    const size_t total = off_w + ws_total;
+    rc = hip_status(hipMalloc(&b->d_base, total));
+    if (rc != BCSR_OK) {
+        delete b;
+        return rc;
+    }
+    char *base = static_cast<char *>(b->d_base);
+    // FP8: point each problem at its slice of the quantised-B workspace
+    for (int i = 0, jn = 0; i < count && fp8; ++i) {
+        if (!mats[i].a.b_chunks) continue;
+        uint8_t *B8 = reinterpret_cast<uint8_t *>(base + off_w + ws_off[i]);
+        uint8_t *sc = B8 + align256(static_cast<size_t>(plans[i]->info.cols) * N[i]);
+        mats[i].a.B8 = B8;
+        mats[i].a.b8scale = sc;
+        jobs[jn].B8 = B8;
+        jobs[jn].scale = sc;
+        ++jn;
+    }
+    std::vector<int4> witems(items.size());
+    for (size_t k = 0; k < items.size(); ++k) witems[k] = items[k].w;
+    struct Copy {
+        size_t off;
+        const void *src;
+        size_t bytes;
+    } copies[] = {{off_m, mats.data(), sizeof(BatchMat) * count},
+                  {off_i, witems.data(), sizeof(int4) * witems.size()},
+                  {off_j, jobs.data(), sizeof(QuantJob) * jobs.size()}};
+    for (const Copy &c : copies) {
+        if (!c.bytes) continue;
+        rc = hip_status(hipMemcpy(base + c.off, c.src, c.bytes, hipMemcpyHostToDevice));
+        if (rc != BCSR_OK) {
+            (void)hipFree(b->d_base);
+            delete b;
+            return rc;
+        }
+    }
+    b->device = dev;
+    b->precision = precision;
+    b->count = count;
+    b->n_items = static_cast<int>(items.size());
+    b->grid = grid;
+    b->n_jobs = static_cast<int>(jobs.size());
+    b->quant_wg = quant_wg;
+    b->quant_grid = std::max(1, std::min(quant_wg, n_cu * 8));
+    b->flops = flops;
+    b->bytes = total;
+    b->d_mats = reinterpret_cast<BatchMat *>(base + off_m);
+    b->d_items = reinterpret_cast<int4 *>(base + off_i);
+    b->d_counter = reinterpret_cast<int *>(base + off_c);
+    b->d_jobs = reinterpret_cast<QuantJob *>(base + off_j);
+    b->tiles = std::move(tiles);
+    *out = b;
+    return BCSR_OK;
+}
+
+extern "C" int bcsr_batch_run(bcsr_batch *batch, void *stream) {
+    if (!batch) return BCSR_ERR_INVALID_ARG;
+    return batch_enqueue(batch, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int bcsr_batch_run_graph(bcsr_batch *batch, void *stream) {
+    if (!batch || !stream) return BCSR_ERR_INVALID_ARG;  // the null stream cannot be captured
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!batch->exec) {
+        int rc = hip_status(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        if (rc != BCSR_OK) return rc;
+        rc = batch_enqueue(batch, s);
+        hipGraph_t g = nullptr;
+        const int rc_end = hip_status(hipStreamEndCapture(s, &g));
+        if (rc != BCSR_OK || rc_end != BCSR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != BCSR_OK ? rc : rc_end;
+        }
+        rc = hip_status(hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0));
+        if (rc != BCSR_OK) {
+            (void)hipGraphDestroy(g);
+            batch->exec = nullptr;
+            return rc;
+        }
+        batch->graph = g;
+    }
+    return hip_status(hipGraphLaunch(batch->exec, s));
+}
+
+extern "C" int bcsr_batch_get_info(const bcsr_batch *batch, bcsr_batch_info *info) {
+    if (!batch || !info) return BCSR_ERR_INVALID_ARG;
+    *info = bcsr_batch_info{};
+    info->count = batch->count;
+    info->items = batch->n_items;
+    info->grid = batch->grid;
+    info->precision = batch->precision;
+    info->flops = batch->flops;
+    info->device_bytes = static_cast<int64_t>(batch->bytes);
+    return BCSR_OK;
+}
+
+extern "C" int bcsr_batch_tiles(const bcsr_batch *batch, int32_t *tiles) {
+    if (!batch || (!tiles && batch->count)) return BCSR_ERR_INVALID_ARG;
+    std::copy(batch->tiles.begin(), batch->tiles.end(), tiles);
+    return BCSR_OK;
+}
+
+extern "C" int bcsr_batch_destroy(bcsr_batch *batch) {
+    if (!batch) return BCSR_OK;
+    int rc = BCSR_OK;
+    if (batch->exec) rc = hip_status(hipGraphExecDestroy(batch->exec));
+    if (batch->graph) {
+        const int r2 = hip_status(hipGraphDestroy(batch->graph));
+        if (rc == BCSR_OK) rc = r2;
+    }
+    if (batch->d_base) {
+        const int r3 = hip_status(hipFree(batch->d_base));
+        if (rc == BCSR_OK) rc = r3;
+    }
+    delete batch;
+    return rc;
+}
