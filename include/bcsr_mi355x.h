@@ -93,6 +93,11 @@ int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N,
 /* reps back-to-back launches of the same product on one stream (timing). */
 int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N,
                      const bcsr_launch_cfg *cfg, int32_t reps, void *stream);
+/* Average time of `reps` launches after `warmup` untimed ones, measured with
+ * HIP events on `stream` (synchronises the stream). */
+int bcsr_spmm_time(const bcsr_plan *plan, const void *B, float *C, int32_t N,
+                   const bcsr_launch_cfg *cfg, int32_t warmup, int32_t reps, void *stream,
+                   float *ms);
 /* The FP8 mode's B quantiser, on its own: B (device, K x N fp16) ->
  * B8 (device, K * N bytes, 16-byte aligned), stored k-minor: byte
  * (s * N + n) * 16 + r holds E4M3(B[16 s + r][n] / scale) -- and scales
@@ -132,9 +137,40 @@ int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float
                       const bcsr_batch_cfg *cfg, bcsr_batch **out);
 int bcsr_batch_run(bcsr_batch *batch, void *stream);
 int bcsr_batch_run_graph(bcsr_batch *batch, void *stream);
+/* Average time of one batch run (graph != 0: graph replay) over reps runs
+ * after warmup untimed ones, HIP events on `stream`. */
+int bcsr_batch_time(bcsr_batch *batch, int32_t graph, int32_t warmup, int32_t reps, void *stream,
+                    float *ms);
 int bcsr_batch_get_info(const bcsr_batch *batch, bcsr_batch_info *info);
 int bcsr_batch_tiles(const bcsr_batch *batch, int32_t *tiles);
 int bcsr_batch_destroy(bcsr_batch *batch);
+
+/* ---- learned dispatch: the cost model's MLP ------------------------------------
+ * No reference counterpart.  A 3-layer perceptron in -> hidden -> hidden ->
+ * out (GELU after the two hidden layers, fp32) trained on the GPU by this
+ * library's own forward, backward and Adam kernels.  Parameters are one flat
+ * fp32 array W1[hidden][in] b1 W2[hidden][hidden] b2 W3[out][hidden] b3;
+ * create initialises W ~ U(-1/sqrt(fan_in), +) from mt19937(seed), b = 0.
+ * Activations X [n][in], targets Y and mask M [n][out] are device arrays;
+ * the loss is sum M (P - Y)^2 * inv_count (inv_count = 1 / sum M). */
+typedef struct bcsr_costmodel bcsr_costmodel;
+int bcsr_cm_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint32_t seed,
+                   bcsr_costmodel **out);
+int bcsr_cm_destroy(bcsr_costmodel *cm);
+int bcsr_cm_num_params(const bcsr_costmodel *cm, int32_t *n);
+int bcsr_cm_set_params(bcsr_costmodel *cm, const float *host, int32_t n); /* resets Adam */
+int bcsr_cm_get_params(const bcsr_costmodel *cm, float *host, int32_t n);
+int bcsr_cm_get_grads(const bcsr_costmodel *cm, float *host, int32_t n);
+/* P [n][out] = model(X). */
+int bcsr_cm_forward(bcsr_costmodel *cm, const float *X, int32_t n, float *P, void *stream);
+/* Gradient of the loss at the current parameters (read with get_grads);
+ * *loss (optional) = the loss, which synchronises the stream. */
+int bcsr_cm_backward(bcsr_costmodel *cm, const float *X, const float *Y, const float *M, int32_t n,
+                     float inv_count, float *loss, void *stream);
+/* steps full-batch Adam steps; *loss (optional) = loss before the last step. */
+int bcsr_cm_train(bcsr_costmodel *cm, const float *X, const float *Y, const float *M, int32_t n,
+                  float inv_count, int32_t steps, float lr, float beta1, float beta2, float eps,
+                  float *loss, void *stream);
 
 /* ---- drop-in device arrays -------------------------------------------------
  * Replaces BCSRMatrix::copyToDevice (formats/BCSRMatrix.cu:106-132): builds an

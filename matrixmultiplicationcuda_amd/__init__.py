@@ -123,6 +123,8 @@ def _declare(lib):
     f("bcsr_spmm", i, _P, _P, _P, _I32, _P)
     f("bcsr_spmm_cfg", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _P)
     f("bcsr_spmm_repeat", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _I32, _P)
+    f("bcsr_spmm_time", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _I32, _I32, _P,
+      ctypes.POINTER(ctypes.c_float))
     f("bcsr_spmm_kernel_name", i, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), ctypes.c_char_p,
       ctypes.c_size_t)
     f("bcsr_batch_tile_supported", i, _I32, _I32, _I32)
@@ -130,9 +132,20 @@ def _declare(lib):
       ctypes.POINTER(_P))
     f("bcsr_batch_run", i, _P, _P)
     f("bcsr_batch_run_graph", i, _P, _P)
+    f("bcsr_batch_time", i, _P, _I32, _I32, _I32, _P, ctypes.POINTER(ctypes.c_float))
     f("bcsr_batch_get_info", i, _P, ctypes.POINTER(bcsr_batch_info))
     f("bcsr_batch_tiles", i, _P, _P)
     f("bcsr_batch_destroy", i, _P)
+    f("bcsr_cm_create", i, _I32, _I32, _I32, ctypes.c_uint32, ctypes.POINTER(_P))
+    f("bcsr_cm_destroy", i, _P)
+    f("bcsr_cm_num_params", i, _P, _I32P)
+    for name in ("bcsr_cm_set_params", "bcsr_cm_get_params", "bcsr_cm_get_grads"):
+        f(name, i, _P, _P, _I32)
+    f("bcsr_cm_forward", i, _P, _P, _I32, _P, _P)
+    f("bcsr_cm_backward", i, _P, _P, _P, _P, _I32, ctypes.c_float,
+      ctypes.POINTER(ctypes.c_float), _P)
+    f("bcsr_cm_train", i, _P, _P, _P, _P, _I32, ctypes.c_float, _I32, ctypes.c_float,
+      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _P)
     f("bcsr_copy_to_device", i, _P, _P, _P, _I32, _I32, _I32, _I32, ctypes.POINTER(_P),
       ctypes.POINTER(_P), ctypes.POINTER(_P))
     f("bcsr_release_device", i, _P)
@@ -478,6 +491,20 @@ class Plan:
                                          ctypes.byref(cfg) if cfg is not None else None, reps,
                                          stream_handle(stream)), "bcsr_spmm_repeat")
 
+    def time_ptr(self, B_ptr, C_ptr, N, stream=None, cfg=None, warmup=2, reps=5) -> float:
+        """Average milliseconds of one launch (HIP events, synchronises)."""
+        ms = ctypes.c_float(0.0)
+        check(self._lib.bcsr_spmm_time(self._handle, B_ptr, C_ptr, N,
+                                       ctypes.byref(cfg) if cfg is not None else None, warmup,
+                                       reps, stream_handle(stream), ctypes.byref(ms)),
+              "bcsr_spmm_time")
+        return ms.value
+
+    def supports(self, N, cfg) -> bool:
+        """Whether a launch configuration is available for this plan and N."""
+        buf = ctypes.create_string_buffer(256)
+        return self._lib.bcsr_spmm_kernel_name(self._handle, N, ctypes.byref(cfg), buf, 256) == 0
+
     def spmm(self, B: DeviceArray, C: DeviceArray = None, stream=None, cfg=None) -> DeviceArray:
         """C = A . B on the GPU (B fp16 K x N, C fp32 M x N DeviceArrays)."""
         if B.dtype != np.float16 or len(B.shape) != 2 or B.shape[0] != self.cols:
@@ -555,6 +582,13 @@ class Batch:
                   "bcsr_batch_run_graph")
         else:
             check(self._lib.bcsr_batch_run(self._handle, stream_handle(stream)), "bcsr_batch_run")
+
+    def time(self, stream=None, graph=False, warmup=2, reps=5) -> float:
+        """Average milliseconds of one batch run (HIP events, synchronises)."""
+        ms = ctypes.c_float(0.0)
+        check(self._lib.bcsr_batch_time(self._handle, int(graph), warmup, reps,
+                                        stream_handle(stream), ctypes.byref(ms)), "bcsr_batch_time")
+        return ms.value
 
     def close(self):
         if self._handle:

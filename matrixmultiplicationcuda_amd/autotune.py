@@ -1,0 +1,136 @@
+"""Autotuner: exhaustive timing of launch configurations, per matrix and per
+batch, with a persistent timing database.
+
+Per matrix the candidate set is CONFIGS (output tile width x kernel variant
+for the 16-bit modes, tile width for FP8); a configuration the library does
+not build for a plan (bcsr_spmm_kernel_name refuses it) is recorded as
+missing.  Per batch the tuner picks each problem's tile width from its own
+timings (restricted to the widths the persistent kernel is compiled for) and
+then sweeps the persistent grid (waves per CU) and the work-item size.
+
+The database is a JSON-lines file, one record per tuned matrix: its
+features, N, the measured milliseconds of every configuration (null where
+unsupported) and of the library's automatic choice.  Appending is the
+checkpoint: a run that stops can be resumed from the records already there.
+"""
+import json
+import os
+
+import numpy as np
+
+from . import Batch, check, lib, make_cfg
+
+TILES = (32, 64, 128, 256)
+CONFIGS_16 = tuple((t, v) for t in TILES for v in (0, 1))  # (tile_n, variant)
+CONFIGS_FP8 = tuple((t, -1) for t in TILES)
+BATCH_WAVES = (4, 8, 12, 16)
+BATCH_BINS = (32, 64, 128, 256)
+
+
+def configs_for(precision):
+    return CONFIGS_FP8 if precision == "fp8" else CONFIGS_16
+
+
+def cfg_of(config):
+    tile, variant = config
+    return make_cfg(tile_n=tile, variant=variant, xcd_map=-1, waves_per_block=1)
+
+
+def supported_mask(plan, N, configs):
+    return np.array([plan.supports(N, cfg_of(c)) for c in configs], dtype=bool)
+
+
+def time_configs(plan, B, C, N, configs, stream=None, warmup=2, reps=5):
+    """Milliseconds per launch of every configuration (nan if unsupported)."""
+    out = np.full(len(configs), np.nan)
+    for k, c in enumerate(configs):
+        cfg = cfg_of(c)
+        if plan.supports(N, cfg):
+            out[k] = plan.time_ptr(B.ptr, C.ptr, N, stream, cfg, warmup, reps)
+    return out
+
+
+def time_auto(plan, B, C, N, stream=None, warmup=2, reps=5):
+    return plan.time_ptr(B.ptr, C.ptr, N, stream, None, warmup, reps)
+
+
+class TimingDB:
+    """JSON-lines store of per-matrix timings (append-only)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def load(self):
+        if not self.path or not os.path.exists(self.path):
+            return []
+        with open(self.path) as f:
+            return [json.loads(line) for line in f if line.strip()]
+
+    def keys(self):
+        return {r["key"] for r in self.load()}
+
+    def append(self, record):
+        if not self.path:
+            return
+        d = os.path.dirname(self.path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        with open(self.path, "a") as f:
+            f.write(json.dumps(record) + "\n")
+
+
+def record(key, feats, N, precision, configs, times, auto_ms):
+    return {"key": key, "precision": precision, "N": int(N),
+            "features": [float(x) for x in feats],
+            "configs": [list(c) for c in configs],
+            "ms": [None if not np.isfinite(t) else float(t) for t in times],
+            "auto_ms": float(auto_ms)}
+
+
+def batch_tiles_from_times(plans, Ns, times, configs, precision_id):
+    """Each problem's fastest tile width among those the persistent kernel
+    supports (its timings of the pipelined variant, else any variant)."""
+    tiles = []
+    for p, N, t in zip(plans, Ns, times):
+        best, best_ms = None, np.inf
+        dev_bh = p.info.dev_bh
+        for c, ms in zip(configs, t):
+            tile, variant = c
+            if not np.isfinite(ms) or variant == 0:
+                continue
+            if not lib().bcsr_batch_tile_supported(precision_id, dev_bh, tile):
+                continue
+            if ms < best_ms:
+                best, best_ms = tile, ms
+        if best is None:
+            best = 0  # the batch's automatic width
+        tiles.append(best)
+    return tiles
+
+
+def tune_batch(plans, Bs, Cs, tiles, stream, waves=BATCH_WAVES, bins=BATCH_BINS, graph=True,
+               warmup=2, reps=5):
+    """Sweep the persistent grid and item size for fixed per-problem tiles.
+    Returns (best dict, list of all (waves, bin, ms))."""
+    trials = []
+    best = None
+    for w in waves:
+        for b in bins:
+            batch = Batch(plans, Bs, Cs, tiles=tiles, waves_per_cu=w, bin_work=b)
+            try:
+                ms = batch.time(stream, graph=graph, warmup=warmup, reps=reps)
+            finally:
+                batch.close()
+            trials.append((w, b, ms))
+            if best is None or ms < best["ms"]:
+                best = {"waves_per_cu": w, "bin_work": b, "ms": ms, "tiles": list(tiles)}
+    return best, trials
+
+
+def precision_id(precision):
+    from . import PRECISIONS
+    return PRECISIONS[precision]
+
+
+__all__ = ["CONFIGS_16", "CONFIGS_FP8", "TimingDB", "time_configs", "time_auto", "tune_batch",
+           "batch_tiles_from_times", "configs_for", "cfg_of", "supported_mask", "record", "check"]

@@ -485,6 +485,27 @@ int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N, 
     return BCSR_OK;
 }
 
+int bcsr_spmm_time(const bcsr_plan *plan, const void *B, float *C, int32_t N, const bcsr_launch_cfg *cfg,
+                   int32_t warmup, int32_t reps, void *stream, float *ms) {
+    if (!ms || reps <= 0 || warmup < 0) return BCSR_ERR_INVALID_ARG;
+    int rc = bcsr_spmm_repeat(plan, B, C, N, cfg, warmup, stream);
+    if (rc != BCSR_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = hip_status(hipEventCreate(&e0));
+    if (rc == BCSR_OK) rc = hip_status(hipEventCreate(&e1));
+    if (rc == BCSR_OK) rc = hip_status(hipEventRecord(e0, s));
+    if (rc == BCSR_OK) rc = bcsr_spmm_repeat(plan, B, C, N, cfg, reps, stream);
+    if (rc == BCSR_OK) rc = hip_status(hipEventRecord(e1, s));
+    if (rc == BCSR_OK) rc = hip_status(hipEventSynchronize(e1));
+    float t = 0.0f;
+    if (rc == BCSR_OK) rc = hip_status(hipEventElapsedTime(&t, e0, e1));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc == BCSR_OK) *ms = t / static_cast<float>(reps);
+    return rc;
+}
+
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg, char *buf,
                           size_t buflen) {
     if (!plan || !buf || buflen == 0) return BCSR_ERR_INVALID_ARG;

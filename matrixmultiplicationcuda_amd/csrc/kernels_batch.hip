@@ -387,6 +387,30 @@ extern "C" int bcsr_batch_run_graph(bcsr_batch *batch, void *stream) {
     return hip_status(hipGraphLaunch(batch->exec, s));
 }
 
+extern "C" int bcsr_batch_time(bcsr_batch *batch, int32_t graph, int32_t warmup, int32_t reps, void *stream,
+                               float *ms) {
+    if (!batch || !ms || reps <= 0 || warmup < 0) return BCSR_ERR_INVALID_ARG;
+    auto run = [&]() { return graph ? bcsr_batch_run_graph(batch, stream) : bcsr_batch_run(batch, stream); };
+    for (int i = 0; i < warmup; ++i) {
+        int rc = run();
+        if (rc != BCSR_OK) return rc;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = hip_status(hipEventCreate(&e0));
+    if (rc == BCSR_OK) rc = hip_status(hipEventCreate(&e1));
+    if (rc == BCSR_OK) rc = hip_status(hipEventRecord(e0, s));
+    for (int i = 0; i < reps && rc == BCSR_OK; ++i) rc = run();
+    if (rc == BCSR_OK) rc = hip_status(hipEventRecord(e1, s));
+    if (rc == BCSR_OK) rc = hip_status(hipEventSynchronize(e1));
+    float t = 0.0f;
+    if (rc == BCSR_OK) rc = hip_status(hipEventElapsedTime(&t, e0, e1));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc == BCSR_OK) *ms = t / static_cast<float>(reps);
+    return rc;
+}
+
 extern "C" int bcsr_batch_get_info(const bcsr_batch *batch, bcsr_batch_info *info) {
     if (!batch || !info) return BCSR_ERR_INVALID_ARG;
     *info = bcsr_batch_info{};
