@@ -48,6 +48,9 @@ def parse():
                    help="block storage: fp16 (the reference's), bf16, FP8 E4M3 with per-block "
                         "scales, or hi+lo fp16 pairs")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-extensions", action="store_true",
+                   help="skip the precision-mode and batched/learned measurements")
+    p.add_argument("--batch-count", type=int, default=96)
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
     return p.parse_args()
 
@@ -60,6 +63,124 @@ def algorithmic_bytes(nnzb, bh, bw, M, K, N, precision="f16"):
     scales = 4 * nnzb if precision == "fp8" else 0
     return (nnzb * bh * bw * A_BYTES[precision] + scales + 4 * nnzb + 4 * (M // bh + 1)
             + K * N * 2 + M * N * 4)
+
+
+def time_ms(stream, fn, warmup=2, reps=10):
+    import matrixmultiplicationcuda_amd as mm
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = mm.Event(), mm.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def precision_modes(mm, np, h16, h32, B, N, stream, ref_ms):
+    """The target matrix in bf16, FP8 and hi+lo storage (automatic launch)."""
+    out = {}
+    flops = 2.0 * h16.nnzb * h16.bh * h16.bw * N
+    M = h16.block_rows * h16.bh
+    for prec in ("bf16", "fp8", "hilo"):
+        plan = mm.Plan(h32, prec)
+        C = mm.DeviceArray((M, N), np.float32)
+        ms = time_ms(stream, lambda: plan.spmm_ptr(B.ptr, C.ptr, N, stream), reps=20)
+        out[prec] = {"ms": round(ms, 5), "tflops": round(flops / ms / 1e9, 2),
+                     "vs_f16": round(ref_ms / ms, 3), "kernel": plan.kernel_name(N)}
+        C.free()
+        plan.close()
+    return out
+
+
+def batched(mm, np, count, stream):
+    """The batched uneven workload: plain fp16 per-matrix launches (the
+    library's static choice each) against the batched persistent kernel
+    (graph replays), fp16 and FP8.  The batch's tile widths come from the
+    autotuner, which measures the policies -- the batch's automatic widths,
+    the learned dispatcher's per-matrix picks and (FP8) each matrix's own
+    exhaustive autotune -- each with its grid/item-size sweep."""
+    import concurrent.futures as cf
+
+    import oracle
+    from matrixmultiplicationcuda_amd import autotune, learned, workloads
+    probs = workloads.draw_problems(count, base_seed=1000)
+    with cf.ThreadPoolExecutor(max_workers=min(12, os.cpu_count() or 1)) as ex:
+        built = list(ex.map(lambda p: (p.bcsr(np.float32), p.B()), probs))
+    h32s = [b[0] for b in built]
+    h16s = [mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols,
+                        h.bh, h.bw) for h in h32s]
+    Ns = [p.N for p in probs]
+    flops = sum(workloads.flops(h, p.N) for h, p in zip(h16s, probs))
+    Bs = [mm.DeviceArray.from_numpy(b[1]) for b in built]
+    Cs = [mm.DeviceArray((p.rows, p.N), np.float32) for p in probs]
+    res = {"workload": f"{count} problems: rows/cols log-uniform 512..16384 (x64), blocks "
+                       "16/32/64 x 16, block density U(0.5%, 25%), blockrandom_rho U(0.3, 1), "
+                       "N in {32, 128, 512}; seeds 1000 + i",
+           "gflop": round(flops / 1e9, 3)}
+    out = {}
+    for prec, hosts in (("f16", h16s), ("fp8", h32s)):
+        plans = [mm.Plan(h, prec) for h in hosts]
+
+        def seq():
+            for pl, B, C in zip(plans, Bs, Cs):
+                pl.spmm_ptr(B.ptr, C.ptr, B.shape[1], stream)
+        t_seq = time_ms(stream, seq, reps=5)
+        disp = learned.Dispatcher(prec)
+        policies = {"auto": None, "learned": disp.batch_tiles(plans, h16s, Ns)}
+        # per-matrix launches with the learned dispatcher's configurations
+        cfgs = [autotune.cfg_of(disp.choose(pl, h, N)[0]) for pl, h, N in zip(plans, h16s, Ns)]
+        disp.close()
+
+        def seq_learned():
+            for pl, B, C, cfg in zip(plans, Bs, Cs, cfgs):
+                pl.spmm_ptr(B.ptr, C.ptr, B.shape[1], stream, cfg)
+        t_seq_l = time_ms(stream, seq_learned, reps=5)
+        if prec == "fp8":
+            # per-problem exhaustive autotune of the tiles (each matrix alone)
+            configs = autotune.configs_for("fp8")
+            times = [autotune.time_configs(p, B, C, B.shape[1], configs, stream, reps=3)
+                     for p, B, C in zip(plans, Bs, Cs)]
+            policies["per_matrix_autotuned"] = autotune.batch_tiles_from_times(
+                plans, Ns, times, configs, 2)
+        best, table = autotune.tune_batch_policies(plans, Bs, Cs, policies, stream)
+        batch = mm.Batch(plans, Bs, Cs, tiles=best["tiles"], waves_per_cu=best["waves_per_cu"],
+                         bin_work=best["bin_work"])
+        t_b = batch.time(stream, graph=True, warmup=3, reps=20)
+        entry = {"per_matrix_ms": round(t_seq, 4), "per_matrix_tflops": round(flops / t_seq / 1e9, 2),
+                 "per_matrix_learned_ms": round(t_seq_l, 4),
+                 "batched_ms": round(t_b, 4), "batched_tflops": round(flops / t_b / 1e9, 2),
+                 "tile_policy": best["policy"], "waves_per_cu": best["waves_per_cu"],
+                 "bin_work": best["bin_work"],
+                 "policies_ms": {k: round(v["ms"], 4) for k, v in table.items()}}
+        if prec == "fp8":
+            # error of the batched FP8 results against the FP64 product (3 problems)
+            batch.run(stream)
+            stream.synchronize()
+            errs = []
+            for i in sorted(range(count), key=lambda i: probs[i].rows * probs[i].N)[:3]:
+                h = h32s[i]
+                Cg = Cs[i].numpy().astype(np.float64)
+                C64, Ca = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data, h.bh, h.bw,
+                                               built[i][1].astype(np.float32), threads=8)
+                nz = Ca > 0
+                e = np.abs(Cg - C64)[nz] / Ca[nz]
+                errs.append((float(e.max()), float(e.mean())))
+            entry["error_vs_fp64_per_entry"] = {"max": max(e[0] for e in errs),
+                                                "mean": float(np.mean([e[1] for e in errs])),
+                                                "problems": 3}
+        batch.close()
+        for pl in plans:
+            pl.close()
+        out[prec] = entry
+    for a in Bs + Cs:
+        a.free()
+    res.update(out)
+    res["speedup_fp8_batched_vs_fp16_per_matrix"] = round(
+        out["f16"]["per_matrix_ms"] / out["fp8"]["batched_ms"], 3)
+    return res
 
 
 def main():
@@ -178,6 +299,14 @@ def main():
                               "exact_zero_where_absAabsB_zero": exact_zero,
                               "ok": rel <= 1e-3 and exact_zero}}
 
+    extensions = None
+    if not args.no_extensions and args.precision == "f16":
+        t1 = time.time()
+        extensions = {"precision_modes_at_this_workload":
+                      precision_modes(mm, np, h, h32, B, N, stream, ms)}
+        extensions["batched"] = batched(mm, np, args.batch_count, stream)
+        print(f"# extensions {time.time() - t1:.1f}s", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC,
         "value": round(tflops, 3),
@@ -201,6 +330,7 @@ def main():
         "gbs": round(gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "extensions": extensions,
     }
     print(json.dumps(out), flush=True)
     stream.destroy()

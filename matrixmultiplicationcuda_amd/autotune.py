@@ -4,9 +4,11 @@ batch, with a persistent timing database.
 Per matrix the candidate set is CONFIGS (output tile width x kernel variant
 for the 16-bit modes, tile width for FP8); a configuration the library does
 not build for a plan (bcsr_spmm_kernel_name refuses it) is recorded as
-missing.  Per batch the tuner picks each problem's tile width from its own
-timings (restricted to the widths the persistent kernel is compiled for) and
-then sweeps the persistent grid (waves per CU) and the work-item size.
+missing.  Per batch the tuner measures tile-width policies (the batch's
+automatic widths, the learned dispatcher's picks, each problem's own fastest
+width restricted to those the persistent kernel is compiled for), each with a
+sweep of the persistent grid (waves per CU) and the work-item size, and keeps
+the fastest.
 
 The database is a JSON-lines file, one record per tuned matrix: its
 features, N, the measured milliseconds of every configuration (null where
@@ -117,14 +119,30 @@ def tune_batch(plans, Bs, Cs, tiles, stream, waves=BATCH_WAVES, bins=BATCH_BINS,
     for w in waves:
         for b in bins:
             batch = Batch(plans, Bs, Cs, tiles=tiles, waves_per_cu=w, bin_work=b)
+            tiles_used = list(batch.tiles)
             try:
                 ms = batch.time(stream, graph=graph, warmup=warmup, reps=reps)
             finally:
                 batch.close()
             trials.append((w, b, ms))
             if best is None or ms < best["ms"]:
-                best = {"waves_per_cu": w, "bin_work": b, "ms": ms, "tiles": list(tiles)}
+                best = {"waves_per_cu": w, "bin_work": b, "ms": ms, "tiles": tiles_used}
     return best, trials
+
+
+def tune_batch_policies(plans, Bs, Cs, policies, stream, waves=BATCH_WAVES, bins=BATCH_BINS,
+                        graph=True, warmup=2, reps=3):
+    """Tile widths by policy (name -> per-problem tiles, None = the batch's
+    automatic widths), each with its own grid/item sweep; returns the best
+    {policy, tiles, waves_per_cu, bin_work, ms} and the ms of every policy."""
+    table = {}
+    best = None
+    for name, tiles in policies.items():
+        b, _ = tune_batch(plans, Bs, Cs, tiles, stream, waves, bins, graph, warmup, reps)
+        table[name] = {"ms": b["ms"], "waves_per_cu": b["waves_per_cu"], "bin_work": b["bin_work"]}
+        if best is None or b["ms"] < best["ms"]:
+            best = dict(b, policy=name)
+    return best, table
 
 
 def precision_id(precision):
@@ -133,4 +151,5 @@ def precision_id(precision):
 
 
 __all__ = ["CONFIGS_16", "CONFIGS_FP8", "TimingDB", "time_configs", "time_auto", "tune_batch",
-           "batch_tiles_from_times", "configs_for", "cfg_of", "supported_mask", "record", "check"]
+           "batch_tiles_from_times", "tune_batch_policies", "configs_for", "cfg_of",
+           "supported_mask", "record", "check"]

@@ -214,3 +214,57 @@ def regret_report(pred_log, M, ms, auto_ms, spread=FALLBACK_SPREAD):
             "fallback_fraction": fallbacks / max(ms.shape[0], 1),
             "static_mean_regret": float(np.nanmean(static)),
             "static_p90_regret": float(np.nanpercentile(static, 90))}
+
+
+def default_model_path(precision):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                        f"costmodel_{precision}.json")
+
+
+class Dispatcher:
+    """Picks a launch configuration per matrix from the trained cost model;
+    when the predicted spread is under FALLBACK_SPREAD it times every
+    candidate instead (exhaustive autotune) if operands are given."""
+
+    def __init__(self, precision="f16", path=None):
+        self.precision = precision
+        self.model, self.configs, self.meta = CostModel.load(path or default_model_path(precision))
+
+    def predict(self, hosts_and_N):
+        F = np.stack([features(h, N) for h, N in hosts_and_N])
+        return self.model.predict(F)
+
+    def choose(self, plan, h, N, B=None, C=None, stream=None):
+        """(config, used_fallback) for one product."""
+        from . import autotune
+        pred = self.predict([(h, N)])[0]
+        mask = autotune.supported_mask(plan, N, self.configs).astype(np.float32)
+        k, fb = choose(pred, mask)
+        if fb and B is not None and C is not None:
+            t = autotune.time_configs(plan, B, C, N, self.configs, stream)
+            k = int(np.nanargmin(t))
+        return self.configs[k], fb
+
+    def batch_tiles(self, plans, hosts, Ns):
+        """Per-problem tile widths for a batch: the predicted-fastest
+        configuration among those the persistent kernel runs (it always uses
+        the pipelined variant of the 16-bit kernels)."""
+        from . import PRECISIONS
+        prec = PRECISIONS[self.precision]
+        pred = self.predict(list(zip(hosts, Ns)))
+        tiles = []
+        for p, N, row in zip(plans, Ns, pred):
+            dev_bh = p.info.dev_bh
+            best, best_v = 0, np.inf
+            for c, v in zip(self.configs, row):
+                tile, variant = c
+                if variant == 0 or not lib().bcsr_batch_tile_supported(prec, dev_bh, tile):
+                    continue
+                if v < best_v:
+                    best, best_v = tile, v
+            tiles.append(best)
+        return tiles
+
+    def close(self):
+        self.model.close()
