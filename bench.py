@@ -27,8 +27,9 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "fp8": 5000.0, "hilo": 2500.0
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=500,
+                   help="untimed launches first (lets the GPU reach its steady clocks)")
     p.add_argument("--M", type=int, default=16384)
     p.add_argument("--K", type=int, default=16384)
     p.add_argument("--N", type=int, default=512)

@@ -110,7 +110,10 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
 // to XCDs: n_tiles % 8 == 0 gives XCD x the tiles x, x + 8, ...; n_tiles in
 // {1, 2, 4} lets 8 / n_tiles XCDs share each tile.
 __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
-    const int bid = blockIdx.x, w = threadIdx.x >> 6;
+    // threadIdx.x >> 6 is the same in every lane of a wave; readfirstlane
+    // tells the compiler so, which keeps everything derived from it (row,
+    // group range, column indices) in scalar registers and scalar loads
+    const int bid = blockIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (a.map_mode == 0) {
         const int item = bid * a.wpb + w;
         tile = item % a.n_tiles;

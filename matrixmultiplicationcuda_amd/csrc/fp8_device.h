@@ -76,7 +76,10 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, uint8
 // Work item (device block-row br, tile of 16 * EPL columns); MFMA e covers
 // columns tile + 16 e + j for lane j of each lane group.
 template <int RS, int EPL>
-__device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br, int tile) {
+__device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_in) {
+    // wave-uniform by construction (scalar index chain)
+    const int br = __builtin_amdgcn_readfirstlane(br_in);
+    const int tile = __builtin_amdgcn_readfirstlane(tile_in);
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int n_base = tile * (16 * EPL);

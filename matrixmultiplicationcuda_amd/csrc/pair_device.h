@@ -177,7 +177,11 @@ __device__ __forceinline__ void store_tile(const Acc<RS, EPL, PREC> &acc, float 
 // the wait counts are the same on every path and each wait covers exactly one
 // pair's loads.
 template <int RS, int EPL, bool PF, int PREC>
-__device__ __forceinline__ void pair_work(const SpmmArgs &a, int br, int tile) {
+__device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile_in) {
+    // wave-uniform by construction; scalar registers keep the index chain
+    // (group range, column indices, B offsets) off the vector memory path
+    const int br = __builtin_amdgcn_readfirstlane(br_in);
+    const int tile = __builtin_amdgcn_readfirstlane(tile_in);
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int n0 = tile * (16 * EPL) + EPL * j;
