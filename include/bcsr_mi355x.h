@@ -101,9 +101,12 @@ int bcsr_spmm_time(const bcsr_plan *plan, const void *B, float *C, int32_t N,
 /* The FP8 mode's B quantiser, on its own: B (device, K x N fp16) ->
  * B8 (device, K * N bytes, 16-byte aligned), stored k-minor: byte
  * (s * N + n) * 16 + r holds E4M3(B[16 s + r][n] / scale) -- and scales
- * (device, (K / 16) * ceil(N / 256) bytes): the E8M0 exponent of each 16-row
- * slab and 256-column chunk (smallest power of two with |B| / scale <= 448). */
-int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *stream);
+ * (device, ceil(N / 256) bytes): the E8M0 exponent of each 256-column chunk
+ * over all K rows (smallest power of two with |B| / scale <= 448).  work:
+ * device scratch of (K / 16) * ceil(N / 256) floats.  B and B8 16-byte
+ * aligned, N % 16 == 0. */
+int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,
+                         void *stream);
 /* Name of the kernel a launch with this configuration runs (for profiling). */
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
                           char *buf, size_t buflen);

@@ -5,23 +5,20 @@
 //   32 or 16, the largest that divides bh; bw is cut into 16-column pieces).
 //   A device block-row's blocks (ascending block-column) are taken G at a time
 //   (G = 2 "pairs" for the 16-bit modes: one K = 32 step of the 16x16x32 MFMA;
-//   G = 4 "quads" for FP8: one K = 128 step of the scaled 16x16x128 MFMA with
-//   one block per 32-wide K group); short groups are padded with zero blocks
-//   that repeat the last block-column.
+//   G = 8 "octets" for FP8: one K = 128 step of the scaled 16x16x128 MFMA);
+//   short groups are padded with zero blocks that repeat the last column.
 //     grp_ptr[dev_rows + 1]  prefix count of groups per device block-row
 //     grp_col[G * groups]    16-column block index of every group slot
-//     vals                   the A operands, one 16-byte chunk per lane:
+//     vals                   the A operands, 16 bytes per lane and operand:
 //                            pairs: [group][row-subtile s][lane l][8 x 16-bit]
 //                              = rows 16 s + l % 16, columns 8 (l / 16) .. + 7
 //                              of the pair's 16 x 32 slice (hi+lo mode stores
 //                              [group][s][hi, lo][lane]);
-//                            quads: [group][s][lane l < 32][32 x fp8]: bytes
+//                            octets: [group][s][lane l][32 x fp8]: bytes
 //                              16 h .. 16 h + 15 of lane l = row 16 s + l % 16
-//                              of the block in slot 2 h + l / 16 (the MFMA's
-//                              operand for lane group 2 (l / 16); lane groups
-//                              1 and 3 hold zeros and are not stored);
-//     ascale                 FP8 only: per group the four slots' E8M0 block
-//                            scales, byte t = slot t;
+//                              of the block in slot 4 h + l / 16;
+//     ascale                 FP8 only: per (group, s) the E8M0 scales of the
+//                            four slot pairs (2 u, 2 u + 1), byte u;
 //     row_order[dev_rows]    device block-rows by descending group count.
 #pragma once
 
@@ -47,7 +44,7 @@ struct bcsr_plan {
     int device = 0;
     int rs = 1;        // dev_bh / 16
     int dev_rows = 0;  // M / dev_bh
-    int group = 2;     // blocks per group: 2 (pairs) or 4 (FP8 quads)
+    int group = 2;     // blocks per group: 2 (pairs) or 8 (FP8 octets)
     void *dev_base = nullptr;
     int32_t *d_hdr = nullptr;  // reference prefix array (API view)
     int32_t *d_idx = nullptr;  // reference block-column indices (API view)

@@ -130,7 +130,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     if (precision < BCSR_PREC_F16 || precision > BCSR_PREC_F16_HILO) return BCSR_ERR_INVALID_ARG;
 
     const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
-    const int G = fp8 ? 4 : 2;                                  // blocks per group
+    const int G = fp8 ? 8 : 2;                                  // blocks per group
     const int H = precision == BCSR_PREC_F16_HILO ? 2 : 1;      // operand copies (hi, lo)
     const int dev_bh = bh % 64 == 0 ? 64 : (bh % 32 == 0 ? 32 : 16);
     const int rs = dev_bh / 16;
@@ -140,17 +140,6 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     const int64_t nnzb = hdr[block_rows];
     const int64_t blk_elems = static_cast<int64_t>(bh) * bw;
 
-    // FP8: one E8M0 scale per source block
-    std::vector<uint8_t> blk_scale;
-    if (fp8) {
-        blk_scale.resize(nnzb);
-        for (int64_t k = 0; k < nnzb; ++k) {
-            float amax = 0.0f;
-            for (int64_t i = 0; i < blk_elems; ++i) amax = std::max(amax, std::fabs(src.value(k * blk_elems + i)));
-            blk_scale[k] = bcsr_e8m0_for_absmax(amax);
-        }
-    }
-
     std::vector<int32_t> grp_ptr(dev_rows + 1, 0);
     for (int R = 0; R < block_rows; ++R) {
         const int64_t nblk = static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces;
@@ -158,51 +147,74 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     }
     for (int r = 0; r < dev_rows; ++r) grp_ptr[r + 1] += grp_ptr[r];
     const int64_t groups = grp_ptr[dev_rows];
-    const size_t chunk_bytes = 16;                                 // per lane per operand
-    // pairs: 64 lanes x 16 bytes per operand; quads: 32 lanes x 32 bytes
-    const size_t group_bytes = static_cast<size_t>(rs) * H * 64 * chunk_bytes;
+    // pairs: per row-subtile (and hi/lo copy) 64 lanes x 16 bytes;
+    // octets: per row-subtile 64 lanes x 32 bytes
+    const size_t group_bytes = fp8 ? static_cast<size_t>(rs) * 64 * 32 : static_cast<size_t>(rs) * H * 64 * 16;
 
     std::vector<int32_t> grp_col(static_cast<size_t>(G) * groups);
     std::vector<uint8_t> vals(static_cast<size_t>(groups) * group_bytes, 0);
-    std::vector<uint32_t> ascale(fp8 ? groups : 0, 0);
+    std::vector<uint32_t> ascale(fp8 ? static_cast<size_t>(groups) * rs : 0, 0);
     for (int R = 0; R < block_rows; ++R) {
         const int64_t k0 = hdr[R], nblk = static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces;
         for (int t = 0; t < sub_rows; ++t) {
             int64_t p = grp_ptr[R * sub_rows + t];
             for (int64_t b = 0; b < nblk; b += G, ++p) {
                 uint8_t *gv = vals.data() + static_cast<size_t>(p) * group_bytes;
-                for (int slot = 0; slot < G; ++slot) {
-                    int64_t db = b + slot;
-                    const bool pad = db >= nblk;
-                    if (pad) db = nblk - 1;
+                // source offset of element (row, column c) of device block db
+                auto src_index = [&](int64_t db, int row, int c) {
                     const int64_t k = k0 + db / pieces;
                     const int q = static_cast<int>(db % pieces);
-                    grp_col[static_cast<size_t>(G) * p + slot] = idx[k] * pieces + q;
-                    if (fp8) ascale[p] |= static_cast<uint32_t>(pad ? 127 : blk_scale[k]) << (8 * slot);
-                    if (pad) continue;
-                    const float inv = fp8 ? std::ldexp(1.0f, 127 - static_cast<int>(blk_scale[k])) : 1.0f;
+                    return k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + c;
+                };
+                for (int slot = 0; slot < G; ++slot) {
+                    int64_t db = b + slot;
+                    if (db >= nblk) db = nblk - 1;  // padding repeats the last column
+                    const int64_t k = k0 + db / pieces;
+                    grp_col[static_cast<size_t>(G) * p + slot] = idx[k] * pieces + static_cast<int>(db % pieces);
+                }
+                if (fp8) {
+                    // slots (2 u, 2 u + 1) share one E8M0 scale per row-subtile;
+                    // slot -> lane group slot & 3, byte half slot >> 2
+                    for (int s = 0; s < rs; ++s)
+                        for (int u = 0; u < 4; ++u) {
+                            float amax = 0.0f;
+                            for (int m = 0; m < 2; ++m) {
+                                const int64_t db = b + 2 * u + m;
+                                if (db >= nblk) continue;
+                                for (int i = 0; i < 16; ++i)
+                                    for (int c = 0; c < 16; ++c)
+                                        amax = std::max(amax, std::fabs(src.value(src_index(db, t * dev_bh + 16 * s + i, c))));
+                            }
+                            const uint8_t se = bcsr_e8m0_for_absmax(amax);
+                            ascale[static_cast<size_t>(p) * rs + s] |= static_cast<uint32_t>(se) << (8 * u);
+                            const float inv = std::ldexp(1.0f, 127 - static_cast<int>(se));
+                            for (int m = 0; m < 2; ++m) {
+                                const int slot = 2 * u + m;
+                                const int64_t db = b + slot;
+                                if (db >= nblk) continue;
+                                const int g = slot & 3, half = slot >> 2;
+                                for (int i = 0; i < 16; ++i) {
+                                    uint8_t *dst = gv + (static_cast<size_t>(s) * 64 + 16 * g + i) * 32 + 16 * half;
+                                    for (int c = 0; c < 16; ++c)
+                                        dst[c] = bcsr_f32_to_e4m3(src.value(src_index(db, t * dev_bh + 16 * s + i, c)) * inv);
+                                }
+                            }
+                        }
+                    continue;
+                }
+                for (int slot = 0; slot < G; ++slot) {
+                    const int64_t db = b + slot;
+                    if (db >= nblk) continue;
                     for (int s = 0; s < rs; ++s)
                         for (int lane = 0; lane < 64; ++lane) {
                             const int g = lane >> 4, i = lane & 15;
-                            const int row = t * dev_bh + 16 * s + i;
-                            if (fp8) {
-                                // quad slot -> 16 bytes of lane (16 (slot & 1) + i) at
-                                // half slot >> 1; lanes are stored compactly (32 of them)
-                                if (lane >= 32 || g != (slot & 1)) continue;
-                                uint8_t *dst = gv + (static_cast<size_t>(s) * 32 + lane) * 32 + 16 * (slot >> 1);
-                                for (int e = 0; e < 16; ++e) {
-                                    const int64_t si = k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + e;
-                                    dst[e] = bcsr_f32_to_e4m3(src.value(si) * inv);
-                                }
-                                continue;
-                            }
                             if ((g >> 1) != slot) continue;
+                            const int row = t * dev_bh + 16 * s + i;
                             for (int h = 0; h < H; ++h) {
                                 uint16_t *dst = reinterpret_cast<uint16_t *>(
-                                    gv + ((static_cast<size_t>(s) * H + h) * 64 + lane) * chunk_bytes);
+                                    gv + ((static_cast<size_t>(s) * H + h) * 64 + lane) * 16);
                                 for (int e = 0; e < 8; ++e) {
-                                    const int64_t si =
-                                        k * blk_elems + static_cast<int64_t>(row) * bw + q * 16 + (g & 1) * 8 + e;
+                                    const int64_t si = src_index(db, row, (g & 1) * 8 + e);
                                     if (precision == BCSR_PREC_F16) {
                                         dst[e] = src.half_bits(si);
                                     } else if (precision == BCSR_PREC_BF16) {
@@ -353,11 +365,10 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     int tile_n = cfg ? cfg->tile_n : 0;
     if (tile_n == 0) {
         // fastest measured on MI355X at 16384 x 16384, 5% blocks, N = 512:
-        // 128-column tiles for 16/32-row device blocks, 64 for 64-row blocks,
-        // 256 for FP8
+        // 128-column tiles for 16/32-row device blocks, 64 for 64-row blocks
+        // (all storage modes)
         tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
         if (plan->rs == 4 && tile_n == 128) tile_n = 64;
-        if (fp8 && N >= 256) tile_n = 256;
     }
     if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
     // automatic choice: the software-pipelined kernel where its register
@@ -374,7 +385,7 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     // buffer offsets are 32-bit: larger B falls back to 64-bit global loads
     if (!fp8 && static_cast<int64_t>(plan->info.cols) * N * 2 > INT32_MAX) ls.variant = 2;
     if (fp8 && (static_cast<int64_t>(plan->info.cols) * N > INT32_MAX ||
-                plan->info.dev_pairs * plan->rs * 1024 > INT32_MAX))
+                plan->info.dev_pairs * plan->rs * 2048 > INT32_MAX))
         return BCSR_ERR_UNSUPPORTED;
     ls.epl = tile_n / 16;
     ls.n_tiles = (N + tile_n - 1) / tile_n;

@@ -2,22 +2,22 @@
 // v_mfma_scale_f32_16x16x128_f8f6f4 (no reference counterpart: the
 // reference's kernels are fp16 only).
 //
-// Every call first quantises B into a workspace (quant_b_fp8): one E8M0
-// scale per 16-row slab and 256-column chunk, and the values stored k-minor,
-// B8[slab][n][16] (the 16 rows of a slab for column n in 16 consecutive
-// bytes), so that 16 bytes are exactly one lane's half of an MFMA B operand
-// and no transpose is left for the product kernel.
+// The scaled 16x16x128 MFMA reads, per lane, 32 bytes of A (row l % 16) and
+// of B (column l % 16).  K index of byte b in lane group g (lanes 16 g ..
+// 16 g + 15) is 64 (b >> 4) + 16 g + (b & 15), and the MFMA applies one E8M0
+// scale per 32 consecutive K (scale group t = K 32 t .. 32 t + 31 = bytes
+// 16 (t >> 1) .. + 15 of lane groups 2 (t & 1) and 2 (t & 1) + 1), taken from
+// lane 16 t + row (A) / column (B) of the scale operands.
 //
-// The product kernel walks quads of A blocks.  The scaled 16x16x128 MFMA
-// takes one E8M0 scale per row (A) or column (B) for each of four 32-wide K
-// groups; K group t is read from lane group t's scale operand and covers bytes
-// 16 (t >> 1) .. + 15 of lane groups 2 (t & 1) and 2 (t & 1) + 1.  Quad slot t
-// therefore puts its block in those bytes of lane group 2 (t & 1) and zeros in
-// lane group 2 (t & 1) + 1, so the per-block A scale and the per-slab B scale
-// are applied by the MFMA itself at no VALU cost.  Half of every operand is
-// padding: lane groups 1 and 3 read through an out-of-range buffer offset,
-// which returns zeros without memory traffic.  The MFMA pipe is not the bound
-// (fp8 runs at twice the fp16 rate).
+// Layout that follows: an octet is 8 consecutive 16 x 16 device blocks of a
+// device block-row; block slot s sits in lane group s & 3, byte half s >> 2,
+// so slots (2 t, 2 t + 1) form scale group t and share one A scale (the
+// scale of a 16 x 32 tile).  B is quantised per call with one scale per
+// 256-column chunk (uniform over K), so every scale group of the B operand
+// has the same scale and any two slabs can share a group.  B is stored
+// k-minor, B8[slab][n][16]: the 16 rows of slab c for column n are 16
+// consecutive bytes, exactly one lane's half operand.  All 64 lanes carry
+// data: one octet step moves 8 blocks through one MFMA per 16 columns.
 #pragma once
 
 #include "bcsr_internal.h"
@@ -37,44 +37,125 @@ __device__ __forceinline__ unsigned e8m0_for_absmax(float amax) {
     return static_cast<unsigned>(k + 127);
 }
 
-// One workgroup of 256 threads quantises 16-row slab c, 256-column chunk h:
-// thread t takes column n = 256 h + t.
-__device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, uint8_t *__restrict__ B8,
-                                           uint8_t *__restrict__ scale, int N, int chunks, int c, int h) {
-    __shared__ float wmax[4];
+// Quantiser geometry: one 128-thread workgroup per (pair of 16-row slabs
+// 2 x, 2 x + 1, 256-column chunk h).  Thread t takes row quad m = t & 3 and
+// the 8 columns 256 h + 8 (t >> 2) .. + 7 of both slabs: eight 16-byte loads
+// issued together, each instruction reading 1 KB (4 rows x 256 contiguous
+// bytes).  N % 16 == 0, so a thread's 8 columns are all in or all out.
+struct SlabPart {
+    u32x4 v[2][4];  // slab 2 x + u, row 4 m + k: columns n .. n + 7
+    int n, m;
+    bool ok, ok1;   // columns exist; second slab exists
+};
+
+__device__ __forceinline__ void load_slab_part(const _Float16 *__restrict__ B, int N, int slabs, int x, int h,
+                                               SlabPart &sp) {
     const int t = threadIdx.x;
-    const int n = h * kChunk + t;
-    const bool valid = n < N;
-    float v[16];
-    float amax = 0.0f;
+    sp.m = t & 3;
+    sp.n = h * kChunk + 8 * (t >> 2);
+    sp.ok = sp.n < N;
+    sp.ok1 = 2 * x + 1 < slabs;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        v[r] = valid ? static_cast<float>(B[static_cast<size_t>(c * 16 + r) * N + n]) : 0.0f;
-        amax = fmaxf(amax, fabsf(v[r]));
-    }
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m, 64));
-    if ((t & 63) == 0) wmax[t >> 6] = amax;
-    __syncthreads();
-    amax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-    const unsigned se = e8m0_for_absmax(amax);
-    if (t == 0) scale[static_cast<size_t>(c) * chunks + h] = static_cast<uint8_t>(se);
-    if (!valid) return;
-    const float inv = ldexpf(1.0f, 127 - static_cast<int>(se));
-    u32x4 q;
-    unsigned w[4];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        int p = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d] * inv, v[4 * d + 1] * inv, 0, false);
-        p = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * d + 2] * inv, v[4 * d + 3] * inv, p, true);
-        w[d] = static_cast<unsigned>(p);
-    }
-    q = u32x4{w[0], w[1], w[2], w[3]};
-    *reinterpret_cast<u32x4 *>(B8 + (static_cast<size_t>(c) * N + n) * 16) = q;
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = sp.ok && (u == 0 || sp.ok1);
+            sp.v[u][k] = ok ? *reinterpret_cast<const u32x4 *>(
+                                  B + static_cast<size_t>((2 * x + u) * 16 + 4 * sp.m + k) * N + sp.n)
+                            : u32x4{0u, 0u, 0u, 0u};
+        }
 }
 
-// Work item (device block-row br, tile of 16 * EPL columns); MFMA e covers
-// columns tile + 16 e + j for lane j of each lane group.
+__device__ __forceinline__ float half_at(const u32x4 &v, int e) {
+    const unsigned d = v[e >> 1];
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>((e & 1) ? d >> 16 : d & 0xffffu)));
+}
+
+// Maximum over a 64-lane wave, then over the two waves of the workgroup
+// (every thread gets it).
+__device__ __forceinline__ float block_max128(float v, float *wmax) {
+#pragma unroll
+    for (int b = 32; b >= 1; b >>= 1) v = fmaxf(v, __shfl_xor(v, b, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    v = fmaxf(wmax[0], wmax[1]);
+    __syncthreads();  // wmax is reused by the next call
+    return v;
+}
+
+// Pass 1: absmax of (slab, chunk h) for slabs 2 x, 2 x + 1 -> part[slab * chunks + h].
+__device__ __forceinline__ void absmax_slab(const _Float16 *__restrict__ B, float *__restrict__ part, int N,
+                                            int chunks, int slabs, int x, int h) {
+    __shared__ float wmax[2];
+    SlabPart sp;
+    load_slab_part(B, N, slabs, x, h, sp);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        float m = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(half_at(sp.v[u][k], e)));
+        m = block_max128(m, wmax);
+        if (threadIdx.x == 0 && (u == 0 || sp.ok1)) part[static_cast<size_t>(2 * x + u) * chunks + h] = m;
+    }
+}
+
+// Pass 2: slabs 2 x, 2 x + 1 of chunk h quantised.  The chunk's scale comes
+// from every slab's partial (the slab-pair 0 workgroup stores it).  Each
+// thread turns its 4 rows x 8 columns into 8 dwords (4 rows of one column
+// each) and puts them in shared memory in the k-minor record order (column
+// record p(c) = c ^ ((c >> 3) & 7) spreads the writes over the banks); thread
+// t then writes the 16-byte records of columns t and t + 128, so the
+// workgroup's global stores are contiguous.
+__device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, const float *__restrict__ part,
+                                           uint8_t *__restrict__ B8, uint8_t *__restrict__ scale, int N, int chunks,
+                                           int slabs, int x, int h) {
+    __shared__ float wmax[2];
+    __shared__ unsigned rec[2][kChunk * 4];
+    SlabPart sp;
+    load_slab_part(B, N, slabs, x, h, sp);
+    float am = 0.0f;
+    for (int k = threadIdx.x; k < slabs; k += 128) am = fmaxf(am, part[static_cast<size_t>(k) * chunks + h]);
+    am = block_max128(am, wmax);
+    const unsigned se = e8m0_for_absmax(am);
+    if (x == 0 && threadIdx.x == 0) scale[h] = static_cast<uint8_t>(se);
+    const float inv = ldexpf(1.0f, 127 - static_cast<int>(se));
+    const int g8 = threadIdx.x >> 2;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][0], e) * inv, half_at(sp.v[u][1], e) * inv, 0,
+                                                    false);
+            q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][2], e) * inv, half_at(sp.v[u][3], e) * inv, q, true);
+            const int col = 8 * g8 + e;
+            const int p = col ^ ((col >> 3) & 7);
+            rec[u][4 * p + sp.m] = static_cast<unsigned>(q);
+        }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (u == 1 && !sp.ok1) break;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int col = threadIdx.x + 128 * half;
+            const int p = col ^ ((col >> 3) & 7);
+            const int n = h * kChunk + col;
+            if (n < N)
+                *reinterpret_cast<u32x4 *>(B8 + (static_cast<size_t>(2 * x + u) * N + n) * 16) =
+                    u32x4{rec[u][4 * p], rec[u][4 * p + 1], rec[u][4 * p + 2], rec[u][4 * p + 3]};
+        }
+    }
+    __syncthreads();  // rec is reused by the next call
+}
+
+// Work item (device block-row br, tile of 16 * EPL columns).  MFMA e covers
+// columns tile + 16 e + j for lane j of each lane group.  The B loads of the
+// row's octets are issued half an octet ahead of the MFMAs that use them
+// (two half-octet buffers), A one octet ahead, so loads stay in flight across
+// octet boundaries.  Past the row's end loads aim at an out-of-range offset
+// (zeros, no traffic), so every path has the same wait counts.
 template <int RS, int EPL>
 __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_in) {
     // wave-uniform by construction (scalar index chain)
@@ -83,10 +164,11 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, j = lane & 15;
     const int n_base = tile * (16 * EPL);
-    const int chunk = n_base / kChunk;
     const unsigned N = static_cast<unsigned>(a.N);
     const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(a.B8, a.b_bytes);
+    const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
     const unsigned lane_off = static_cast<unsigned>(n_base + j) * 16u;
+    const int sb = a.b8scale[n_base / kChunk];
 
     f32x4 acc[RS][EPL];
 #pragma unroll
@@ -95,57 +177,106 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
-    const bool data_lane = (g & 1) == 0;       // lane groups 0 and 2 carry blocks
-    const bool full_tile = n_base + 16 * EPL <= a.N;
-    const int lane32 = (g >> 1) * 16 + j;      // index among the stored lanes
-    const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
-    for (int q = q0; q < q1; ++q) {
-        const int4 cq = reinterpret_cast<const int4 *>(a.grp_col)[q];
-        const int c_scale = g == 0 ? cq.x : (g == 1 ? cq.y : (g == 2 ? cq.z : cq.w));
-        const int c_lo = (g >> 1) ? cq.y : cq.x;  // slots 2 (g >> 1) .. for halves 0 / 1
-        const int c_hi = (g >> 1) ? cq.w : cq.z;
-        const int sa = static_cast<int>((a.ascale[q] >> (8 * g)) & 0xffu);
-        const int sb = a.b8scale[static_cast<size_t>(c_scale) * a.b_chunks + chunk];
-        const unsigned base_lo = static_cast<unsigned>(c_lo) * (16u * N) + lane_off;
-        const unsigned base_hi = static_cast<unsigned>(c_hi) * (16u * N) + lane_off;
-        u32x4 blo[EPL], bhi[EPL];
-        if (full_tile) {
-            // every column of the tile exists: step columns through soffset
-            const unsigned off_lo = data_lane ? base_lo : kOutOfRange;
-            const unsigned off_hi = data_lane ? base_hi : kOutOfRange;
+    if (q0 < q1) {
+        // slab offsets of this lane's two halves (slots g and 4 + g) in octet q;
+        // the eight columns come in as two scalar 16-byte loads (clamped to the
+        // last octet) and are picked per lane group by selects, not branches
+        auto bases = [&](int q, unsigned &lo, unsigned &hi) {
+            const int qc = q < q1 ? q : q1 - 1;
+            const int4 c03 = reinterpret_cast<const int4 *>(a.grp_col)[2 * qc];
+            const int4 c47 = reinterpret_cast<const int4 *>(a.grp_col)[2 * qc + 1];
+            int cl = c03.x, ch = c47.x;
+            cl = g == 1 ? c03.y : cl;
+            ch = g == 1 ? c47.y : ch;
+            cl = g == 2 ? c03.z : cl;
+            ch = g == 2 ? c47.z : ch;
+            cl = g == 3 ? c03.w : cl;
+            ch = g == 3 ? c47.w : ch;
+            lo = q < q1 ? static_cast<unsigned>(cl) * (16u * N) + lane_off : kOutOfRange;
+            hi = q < q1 ? static_cast<unsigned>(ch) * (16u * N) + lane_off : kOutOfRange;
+        };
+        // both halves land in one 8-register operand (no copies before the
+        // MFMA).  Only voffset is range-checked, so column offsets go there and
+        // columns past N (ragged last tile) read as zeros; no branches, so the
+        // loads stay where the schedule puts them.
+        auto load_b = [&](unsigned lo, unsigned hi, int e, u32x8 &b) {
+            const bool ok = n_base + 16 * e + j < a.N;
+            const unsigned ol = ok && lo != kOutOfRange ? lo + 256u * e : kOutOfRange;
+            const unsigned oh = ok && hi != kOutOfRange ? hi + 256u * e : kOutOfRange;
+            b.lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ol), 0, 0);
+            b.hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(oh), 0, 0);
+        };
+        auto load_a = [&](int q, u32x8 (&av)[RS], int (&sa)[RS]) {
 #pragma unroll
-            for (int e = 0; e < EPL; ++e) {
-                blo[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off_lo), 256 * e, 0);
-                bhi[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(off_hi), 256 * e, 0);
+            for (int s = 0; s < RS; ++s) {
+                const unsigned aoff =
+                    q < q1 ? static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32) : kOutOfRange;
+                av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
+                av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
+                const int qc = q < q1 ? q : q1 - 1;
+                const unsigned sc = a.ascale[static_cast<size_t>(qc) * RS + s];
+                sa[s] = static_cast<int>((sc >> (8 * g)) & 0xffu);
             }
-        } else {
-            // ragged last tile: the range check covers voffset only, so every
-            // column offset goes there and missing columns read as zeros
+        };
+
+        // B for half an octet (EPL / 2 column blocks) per step; steps
+        // alternate between two buffers: the loads of step u + 1 are issued
+        // before the MFMAs of step u (the schedule barriers keep them there).
+        // Two octets per loop iteration give every buffer a fixed role, so no
+        // loaded register is copied (a copy would wait for its load).
+        constexpr int HB = EPL / 2 > 0 ? EPL / 2 : 1;
+        u32x8 bufX[HB], bufY[HB];
+        u32x8 avA[RS], avB[RS];
+        int saA[RS], saB[RS];
+        unsigned lo_c, hi_c, lo_n, hi_n;
+        bases(q0, lo_c, hi_c);
+        bases(q0 + 1, lo_n, hi_n);
+        auto issue_half = [&](unsigned lo, unsigned hi, int half, u32x8 (&buf)[HB]) {
 #pragma unroll
-            for (int e = 0; e < EPL; ++e) {
-                const bool ok = data_lane && n_base + 16 * e + j < a.N;
-                const unsigned ol = ok ? base_lo + 256u * e : kOutOfRange;
-                const unsigned oh = ok ? base_hi + 256u * e : kOutOfRange;
-                blo[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ol), 0, 0);
-                bhi[e] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(oh), 0, 0);
+            for (int k = 0; k < HB; ++k) load_b(lo, hi, half * HB + k, buf[k]);
+        };
+        auto compute_half = [&](int half, const u32x8 (&buf)[HB], const u32x8 (&av)[RS], const int (&sa)[RS]) {
+#pragma unroll
+            for (int k = 0; k < HB; ++k) {
+                const int e = half * HB + k;
+                if (e >= EPL) continue;
+                const i32x8 bv = __builtin_bit_cast(i32x8, buf[k]);
+#pragma unroll
+                for (int s = 0; s < RS; ++s)
+                    acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                        __builtin_bit_cast(i32x8, av[s]), bv, acc[s][e], 0, 0, 0, sa[s], 0, sb);
             }
-        }
-        i32x8 av[RS];
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-            const unsigned aoff = data_lane ? static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 32 + lane32) * 32)
-                                            : kOutOfRange;
-            const u32x4 x0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
-            const u32x4 x1 = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
-            av[s] = __builtin_bit_cast(i32x8, (u32x8{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w}));
-        }
-#pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            const i32x8 bv = __builtin_bit_cast(
-                i32x8, (u32x8{blo[e].x, blo[e].y, blo[e].z, blo[e].w, bhi[e].x, bhi[e].y, bhi[e].z, bhi[e].w}));
-#pragma unroll
-            for (int s = 0; s < RS; ++s)
-                acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av[s], bv, acc[s][e], 0, 0, 0, sa, 0, sb);
+        };
+        auto advance = [&](int q) {
+            lo_c = lo_n;
+            hi_c = hi_n;
+            bases(q, lo_n, hi_n);
+        };
+        load_a(q0, avA, saA);
+        issue_half(lo_c, hi_c, 0, bufX);
+        for (int q = q0; q < q1; q += 2) {
+            // octet q (A in avA)
+            issue_half(lo_c, hi_c, 1, bufY);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_half(0, bufX, avA, saA);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(q + 1, avB, saB);
+            issue_half(lo_n, hi_n, 0, bufX);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_half(1, bufY, avA, saA);
+            __builtin_amdgcn_sched_barrier(0);
+            advance(q + 2);
+            // octet q + 1 (A in avB); past the row's end its loads read zeros
+            issue_half(lo_c, hi_c, 1, bufY);
+            __builtin_amdgcn_sched_barrier(0);
+            if (q + 1 < q1) compute_half(0, bufX, avB, saB);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(q + 2, avA, saA);
+            issue_half(lo_n, hi_n, 0, bufX);
+            __builtin_amdgcn_sched_barrier(0);
+            if (q + 1 < q1) compute_half(1, bufY, avB, saB);
+            __builtin_amdgcn_sched_barrier(0);
+            advance(q + 3);
         }
     }
 

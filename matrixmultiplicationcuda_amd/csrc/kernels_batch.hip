@@ -10,8 +10,8 @@
 // device counter (one atomic per item) until the queue is empty, and runs each
 // with the per-matrix kernels' own work functions, so the batched results are
 // bit-identical to the per-matrix launches.  A run is two kernels: a one-thread
-// reset of the counter and the persistent kernel (plus, for FP8, one batched
-// B-quantisation kernel first); the sequence can be captured once into a
+// reset of the counter and the persistent kernel (plus, for FP8, two batched
+// B-quantisation kernels first); the sequence can be captured once into a
 // hipGraph and replayed.  Memory is never cleared with hipMemsetAsync, so the
 // captured graph holds kernel nodes only.
 #include <algorithm>
@@ -34,8 +34,10 @@ struct QuantJob {
     const _Float16 *B;
     uint8_t *B8;
     uint8_t *scale;
+    float *part;  // per (slab, chunk) absmax
     int N;
     int chunks;
+    int slabs;
     int wg0;  // first workgroup index of this job (prefix over slabs * chunks)
 };
 
@@ -97,19 +99,31 @@ __global__ __launch_bounds__(64) void bcsr_batch_persistent(const BatchMat *__re
     }
 }
 
-// FP8: every problem's B quantised in one launch, workgroups striding over
-// the (job, slab, chunk) list.
-__global__ __launch_bounds__(256) void batch_quant(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
+// FP8: every problem's B quantised by two launches (per-slab absmax, then
+// the chunk scales and the quantisation), workgroups striding over the
+// (job, slab pair, chunk) list.
+__device__ __forceinline__ const QuantJob &find_job(const QuantJob *__restrict__ jobs, int n_jobs, int w) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].wg0 <= w) lo = mid; else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+__global__ __launch_bounds__(128) void batch_absmax(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
     for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
-        int lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].wg0 <= w) lo = mid; else hi = mid - 1;
-        }
-        const QuantJob &J = jobs[lo];
+        const QuantJob &J = find_job(jobs, n_jobs, w);
         const int local = w - J.wg0;
-        fp8dev::quant_slab(J.B, J.B8, J.scale, J.N, J.chunks, local / J.chunks, local % J.chunks);
-        __syncthreads();  // the slab reduction's shared memory is reused next iteration
+        fp8dev::absmax_slab(J.B, J.part, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
+    }
+}
+
+__global__ __launch_bounds__(128) void batch_quant(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
+    for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
+        const QuantJob &J = find_job(jobs, n_jobs, w);
+        const int local = w - J.wg0;
+        fp8dev::quant_slab(J.B, J.part, J.B8, J.scale, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
     }
 }
 
@@ -154,7 +168,8 @@ namespace {
 
 int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
     if (b->n_jobs) {
-        hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(256), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
+        hipLaunchKernelGGL(batch_absmax, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
+        hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
         int rc = hip_status(hipGetLastError());
         if (rc != BCSR_OK) return rc;
     }
@@ -217,7 +232,7 @@ extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *cons
             return BCSR_ERR_SHAPE;
         const int64_t kn = static_cast<int64_t>(p->info.cols) * n;
         if (kn * 2 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;  // 32-bit buffer offsets
-        if (fp8 && p->info.dev_pairs * p->rs * 1024 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
+        if (fp8 && p->info.dev_pairs * p->rs * 2048 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
         int t = tile_n && tile_n[i] > 0 ? tile_n[i] : auto_batch_tile(p, n);
         if (t % 16 || !batch_shape(precision, p->rs, t / 16)) return BCSR_ERR_UNSUPPORTED;
         tiles[i] = t;
@@ -240,16 +255,18 @@ extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *cons
         if (fp8 && n && p->info.cols) {
             const int slabs = p->info.cols / 16, chunks = (n + 255) / 256;
             ws_off[i] = ws_total;
-            ws_total += align256(static_cast<size_t>(kn)) + align256(static_cast<size_t>(slabs) * chunks);
+            ws_total += align256(static_cast<size_t>(kn)) + align256(static_cast<size_t>(chunks)) +
+                        align256(sizeof(float) * static_cast<size_t>(slabs) * chunks);
             a.b_bytes = static_cast<int>(kn);
             a.b_chunks = chunks;
-            a.a_bytes = static_cast<int>(p->info.dev_pairs * p->rs * 1024);
+            a.a_bytes = static_cast<int>(p->info.dev_pairs * p->rs * 2048);
             QuantJob J{};
             J.B = static_cast<const _Float16 *>(B[i]);
             J.N = n;
             J.chunks = chunks;
+            J.slabs = slabs;
             J.wg0 = quant_wg;
-            quant_wg += slabs * chunks;
+            quant_wg += (slabs + 1) / 2 * chunks;
             jobs.push_back(J);
         }
     }
@@ -315,10 +332,12 @@ extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *cons
         if (!mats[i].a.b_chunks) continue;
         uint8_t *B8 = reinterpret_cast<uint8_t *>(base + off_w + ws_off[i]);
         uint8_t *sc = B8 + align256(static_cast<size_t>(plans[i]->info.cols) * N[i]);
+        float *part = reinterpret_cast<float *>(sc + align256(static_cast<size_t>(jobs[jn].chunks)));
         mats[i].a.B8 = B8;
         mats[i].a.b8scale = sc;
         jobs[jn].B8 = B8;
         jobs[jn].scale = sc;
+        jobs[jn].part = part;
         ++jn;
     }
     std::vector<int4> witems(items.size());
@@ -346,7 +365,7 @@ extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *cons
     b->grid = grid;
     b->n_jobs = static_cast<int>(jobs.size());
     b->quant_wg = quant_wg;
-    b->quant_grid = std::max(1, std::min(quant_wg, n_cu * 8));
+    b->quant_grid = std::max(1, std::min(quant_wg, n_cu * 16));
     b->flops = flops;
     b->bytes = total;
     b->d_mats = reinterpret_cast<BatchMat *>(base + off_m);

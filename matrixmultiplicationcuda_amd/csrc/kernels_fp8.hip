@@ -10,10 +10,28 @@ namespace {
 
 using namespace fp8dev;
 
-// grid (K / 16, chunks), 256 threads.
-__global__ __launch_bounds__(256) void quant_b_fp8(const _Float16 *__restrict__ B, uint8_t *__restrict__ B8,
-                                                   uint8_t *__restrict__ scale, int N, int chunks) {
-    quant_slab(B, B8, scale, N, chunks, blockIdx.x, blockIdx.y);
+// B quantiser, grid (ceil(slabs / 2), chunks) x 128 threads: per-slab
+// absmax, then the chunk scales and the quantisation.
+__global__ __launch_bounds__(128) void quant_b_absmax(const _Float16 *__restrict__ B, float *__restrict__ part, int N,
+                                                      int chunks, int slabs) {
+    absmax_slab(B, part, N, chunks, slabs, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(128) void quant_b_fp8(const _Float16 *__restrict__ B, const float *__restrict__ part,
+                                                   uint8_t *__restrict__ B8, uint8_t *__restrict__ scale, int N,
+                                                   int chunks, int slabs) {
+    quant_slab(B, part, B8, scale, N, chunks, slabs, blockIdx.x, blockIdx.y);
+}
+
+// work: (K / 16) * chunks floats of partial maxima.
+int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, float *work, hipStream_t stream) {
+    const int slabs = K / 16, chunks = (N + kChunk - 1) / kChunk;
+    const dim3 grid((slabs + 1) / 2, chunks);
+    hipLaunchKernelGGL(quant_b_absmax, grid, dim3(128), 0, stream, static_cast<const _Float16 *>(B), work, N, chunks,
+                       slabs);
+    hipLaunchKernelGGL(quant_b_fp8, grid, dim3(128), 0, stream, static_cast<const _Float16 *>(B), work, B8, scales, N,
+                       chunks, slabs);
+    return hip_status(hipGetLastError());
 }
 
 template <int RS, int EPL>
@@ -58,7 +76,10 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     const int slabs = K / 16;
     const int chunks = (N + kChunk - 1) / kChunk;
     const size_t q_bytes = static_cast<size_t>(K) * N;
-    const size_t need = (q_bytes + 255) / 256 * 256 + static_cast<size_t>(slabs) * chunks;
+    const size_t off_s = (q_bytes + 255) / 256 * 256;
+    const size_t off_p = off_s + 256 * static_cast<size_t>((chunks + 255) / 256);
+    const size_t work_bytes = sizeof(float) * static_cast<size_t>(slabs) * chunks;
+    const size_t need = off_p + work_bytes;
     std::lock_guard<std::mutex> lk(*plan->ws_mu);
     if (plan->ws_bytes < need) {
         if (plan->d_ws) {
@@ -72,10 +93,9 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
         plan->ws_bytes = need;
     }
     uint8_t *B8 = static_cast<uint8_t *>(plan->d_ws);
-    uint8_t *bscale = B8 + (q_bytes + 255) / 256 * 256;
-    hipLaunchKernelGGL(quant_b_fp8, dim3(slabs, chunks), dim3(256), 0, stream, static_cast<const _Float16 *>(B), B8,
-                       bscale, N, chunks);
-    int rc = hip_status(hipGetLastError());
+    uint8_t *bscale = B8 + off_s;
+    float *part = reinterpret_cast<float *>(B8 + off_p);
+    int rc = enqueue_quant(B, K, N, B8, bscale, part, stream);
     if (rc != BCSR_OK) return rc;
     SpmmArgs a{};
     a.grp_ptr = plan->d_grp_ptr;
@@ -92,20 +112,20 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     a.map_mode = ls.map_mode;
     a.b_bytes = static_cast<int>(q_bytes);
     a.b_chunks = chunks;
-    a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 1024, INT32_MAX));
+    a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 2048, INT32_MAX));
     a.wpb = ls.wpb;
     hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
     return hip_status(hipGetLastError());
 }
 
-extern "C" int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *stream) {
+extern "C" int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,
+                                    void *stream) {
     if (K < 0 || N < 0 || K % 16 || N % 16) return BCSR_ERR_SHAPE;
     if (!K || !N) return BCSR_OK;
-    if (!B || !B8 || !scales) return BCSR_ERR_INVALID_ARG;
-    if ((reinterpret_cast<uintptr_t>(B8) & 15u) || (reinterpret_cast<uintptr_t>(B) & 1u)) return BCSR_ERR_SHAPE;
-    const int chunks = (N + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(quant_b_fp8, dim3(K / 16, chunks), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const _Float16 *>(B), static_cast<uint8_t *>(B8), static_cast<uint8_t *>(scales), N,
-                       chunks);
-    return hip_status(hipGetLastError());
+    if (!B || !B8 || !scales || !work) return BCSR_ERR_INVALID_ARG;
+    if ((reinterpret_cast<uintptr_t>(B8) & 15u) || (reinterpret_cast<uintptr_t>(B) & 15u) ||
+        (reinterpret_cast<uintptr_t>(work) & 3u))
+        return BCSR_ERR_SHAPE;
+    return enqueue_quant(B, K, N, static_cast<uint8_t *>(B8), static_cast<uint8_t *>(scales),
+                         static_cast<float *>(work), static_cast<hipStream_t>(stream));
 }

@@ -66,6 +66,7 @@ def lib():
         f("orc_e8m0_for_absmax", ctypes.c_uint8, ctypes.c_float)
         f("orc_quant_e4m3_groups", None, _P, ctypes.c_int64, ctypes.c_int64, _P, _P)
         f("orc_quant_e4m3_B", None, _P, i, i, _P, _P)
+        f("orc_quant_e4m3_tiles", None, _P, _P, _P, i, i, i, _P)
         f("orc_f2bf16", ctypes.c_uint16, ctypes.c_float)
         f("orc_bf162f", ctypes.c_float, ctypes.c_uint16)
         f("orc_hilo_split", None, ctypes.c_float, ctypes.POINTER(ctypes.c_uint16),
@@ -273,13 +274,26 @@ def quant_e4m3_groups(x, group_len):
 
 
 def quant_e4m3_B(B):
-    """(q K x N, scale_exp [K/16, ceil(N/256)]) of the B quantiser."""
+    """(q K x N, scale_exp [ceil(N/256)]) of the B quantiser: one scale per
+    256-column chunk over all rows."""
     B = np.ascontiguousarray(B, dtype=np.float32)
     K, N = B.shape
     q = np.empty((K, N), dtype=np.uint8)
-    sc = np.empty((K // 16, (N + 255) // 256), dtype=np.uint8)
+    sc = np.empty(((N + 255) // 256,), dtype=np.uint8)
     lib().orc_quant_e4m3_B(_p(B), K, N, _p(q), _p(sc))
     return q, sc
+
+
+def fp8_tiles_dequant(hdr, idx, data32, bh, bw):
+    """A as the FP8 mode stores it: 16 x 32 tiles (consecutive 16-column
+    device blocks of a 16-row slice) with one E8M0 scale each, dequantised."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.int32)
+    data = np.ascontiguousarray(data32, dtype=np.float32)
+    out = np.zeros_like(data)
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
+    if data.size:
+        lib().orc_quant_e4m3_tiles(_p(hdr), _p(idx), _p(data), len(hdr) - 1, bh, bw, _p(out))
+    return out
 
 
 def e8m0_to_f32(e):
@@ -307,8 +321,9 @@ def hilo_split(x):
     return hi, lo
 
 
-def dequantized_blocks(mode, data32, bh, bw):
-    """Block values as the storage mode keeps them (fp32-exact)."""
+def dequantized_blocks(mode, data32, bh, bw, hdr=None, idx=None):
+    """Block values as the storage mode keeps them (fp32-exact); FP8 needs
+    the BCSR structure (hdr, idx) because its scales span block pairs."""
     data32 = np.ascontiguousarray(data32, dtype=np.float32)
     if mode == "f16":
         return f16_bits_to_f32(f32_to_f16_bits(data32))
@@ -319,8 +334,7 @@ def dequantized_blocks(mode, data32, bh, bw):
         return (f16_bits_to_f32(hi).astype(np.float64)
                 + f16_bits_to_f32(lo).astype(np.float64) * 2.0**-11)
     if mode == "fp8":
-        q, sc = quant_e4m3_groups(data32, bh * bw)
-        return (e4m3_to_f32(q).reshape(-1, bh * bw) * e8m0_to_f32(sc)[:, None]).reshape(data32.shape)
+        return fp8_tiles_dequant(hdr, idx, data32, bh, bw)
     raise ValueError(mode)
 
 
@@ -334,6 +348,6 @@ def quantized_B(mode, B16_bits):
     if mode == "fp8":
         q, sc = quant_e4m3_B(Bf)
         K, N = Bf.shape
-        scale = np.repeat(np.repeat(e8m0_to_f32(sc), 16, axis=0), 256, axis=1)[:K, :N]
+        scale = np.repeat(e8m0_to_f32(sc), 256)[:N][None, :]
         return e4m3_to_f32(q) * scale
     raise ValueError(mode)

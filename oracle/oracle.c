@@ -601,18 +601,49 @@ void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, u
 
 void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale) {
     int chunks = (N + 255) / 256;
-    for (int c = 0; c < K / 16; ++c)
-        for (int h = 0; h < chunks; ++h) {
-            int n0 = h * 256, n1 = n0 + 256 < N ? n0 + 256 : N;
-            float amax = 0.0f;
-            for (int r = 0; r < 16; ++r)
-                for (int n = n0; n < n1; ++n) amax = fmaxf(amax, fabsf(B[(size_t)(c * 16 + r) * N + n]));
-            uint8_t se = orc_e8m0_for_absmax(amax);
-            scale[(size_t)c * chunks + h] = se;
-            for (int r = 0; r < 16; ++r)
-                for (int n = n0; n < n1; ++n)
-                    q[(size_t)(c * 16 + r) * N + n] = orc_f2e4m3(ldexpf(B[(size_t)(c * 16 + r) * N + n], 127 - (int)se));
-        }
+    for (int h = 0; h < chunks; ++h) {
+        int n0 = h * 256, n1 = n0 + 256 < N ? n0 + 256 : N;
+        float amax = 0.0f;
+        for (int k = 0; k < K; ++k)
+            for (int n = n0; n < n1; ++n) amax = fmaxf(amax, fabsf(B[(size_t)k * N + n]));
+        uint8_t se = orc_e8m0_for_absmax(amax);
+        scale[h] = se;
+        for (int k = 0; k < K; ++k)
+            for (int n = n0; n < n1; ++n)
+                q[(size_t)k * N + n] = orc_f2e4m3(ldexpf(B[(size_t)k * N + n], 127 - (int)se));
+    }
+}
+
+void orc_quant_e4m3_tiles(const int32_t *hdr, const int32_t *idx, const float *data, int block_rows, int bh,
+                          int bw, float *out) {
+    (void)idx;
+    const int pieces = bw / 16;
+    const size_t blk = (size_t)bh * bw;
+    for (int R = 0; R < block_rows; ++R) {
+        const int64_t k0 = hdr[R], nblk = (int64_t)(hdr[R + 1] - hdr[R]) * pieces;
+        for (int t = 0; t < bh / 16; ++t)
+            for (int64_t d = 0; d < nblk; d += 2) {
+                /* device blocks d and d + 1 (16 rows x 16 columns each) share a scale */
+                float amax = 0.0f;
+                for (int64_t e = d; e < d + 2 && e < nblk; ++e) {
+                    const int64_t k = k0 + e / pieces;
+                    const int q = (int)(e % pieces);
+                    for (int i = 0; i < 16; ++i)
+                        for (int c = 0; c < 16; ++c)
+                            amax = fmaxf(amax, fabsf(data[k * blk + (size_t)(16 * t + i) * bw + 16 * q + c]));
+                }
+                const int se = orc_e8m0_for_absmax(amax);
+                for (int64_t e = d; e < d + 2 && e < nblk; ++e) {
+                    const int64_t k = k0 + e / pieces;
+                    const int q = (int)(e % pieces);
+                    for (int i = 0; i < 16; ++i)
+                        for (int c = 0; c < 16; ++c) {
+                            const size_t o = k * blk + (size_t)(16 * t + i) * bw + 16 * q + c;
+                            out[o] = ldexpf(orc_e4m32f(orc_f2e4m3(ldexpf(data[o], 127 - se))), se - 127);
+                        }
+                }
+            }
+    }
 }
 
 /* bfloat16 and hi+lo fp16 storage (extensions). */

@@ -149,9 +149,16 @@ float orc_e4m32f(uint8_t b);
 uint8_t orc_e8m0_for_absmax(float absmax);
 /* Quantise n groups of `len` floats, one E8M0 scale per group. */
 void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, uint8_t *scale);
-/* Quantise B (K x N fp32, row-major): one scale per 16-row slab and
- * 256-column chunk, q row-major like B. */
+/* Quantise B (K x N fp32, row-major): one scale per 256-column chunk over
+ * all K rows, q row-major like B, scale[ceil(N / 256)]. */
 void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale);
+/* The FP8 mode's A: each 16-row slice of a block-row is a sequence of
+ * 16 x 16 device blocks (stored blocks in order, each cut into bw / 16
+ * column pieces); consecutive device blocks (0, 1), (2, 3), ... share one
+ * E8M0 scale (a lone last block has its own).  out = the dequantised values
+ * in the source layout (nnzb row-major bh x bw blocks). */
+void orc_quant_e4m3_tiles(const int32_t *hdr, const int32_t *idx, const float *data, int block_rows, int bh,
+                          int bw, float *out);
 
 /* ---- other storage modes (extensions) ---------------------------------
  * bf16: float -> bfloat16 round to nearest even.  hi+lo: hi = fp16(a),

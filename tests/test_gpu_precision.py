@@ -60,7 +60,7 @@ def test_precision_mode(mode, bh, N):
     h32, data32, B16 = case(M, K, N, bh)
     log(f"mode={mode} bh={bh} N={N} nnzb={h32.nnzb}")
     C = run_mode(h32, B16, mode).astype(np.float64)
-    Aq = oracle.dequantized_blocks(mode, data32, bh, 16)
+    Aq = oracle.dequantized_blocks(mode, data32, bh, 16, h32.hdr, h32.idx)
     Aq32 = np.asarray(Aq, dtype=np.float32)
     assert np.array_equal(Aq32.astype(np.float64), np.asarray(Aq, dtype=np.float64))
     Bq = oracle.quantized_B(mode, B16.view(np.uint16))
@@ -87,21 +87,26 @@ def test_precision_mode(mode, bh, N):
 def test_fp8_b_quantiser_matches_oracle():
     """bcsr_quantize_b_e4m3 == the oracle's quantiser, byte for byte."""
     lib = mm.lib()
-    K, N = 256, 640  # 3 column chunks, the last partial
+    K, N = 4160, 640  # 260 slabs (more than one workgroup's stride), 3 chunks, the last partial
     B16 = mm.generate(K, N, 0.0, "random", seed=11)
     B16[5, 7] = np.float16(0.0)
-    B16[17, :] *= np.float16(1e-3)  # a slab with small values (subnormal codes)
+    B16[17, :] *= np.float16(1e-3)  # small values (subnormal codes)
+    B16[4000:4016, 300:310] = np.float16(3.0)  # the absmax of chunk 1 sits in a late slab
     dB = mm.DeviceArray.from_numpy(B16)
     d8 = mm.DeviceArray((K * N,), np.uint8)
     chunks = (N + 255) // 256
-    ds = mm.DeviceArray((K // 16 * chunks,), np.uint8)
-    mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.ptr, None))
-    mm.synchronize()
-    q_k_minor = d8.numpy().reshape(K // 16, N, 16)
-    got = q_k_minor.transpose(0, 2, 1).reshape(K, N)
+    ds = mm.DeviceArray((chunks,), np.uint8)
+    work = mm.DeviceArray((K // 16 * chunks,), np.float32)  # per-slab partial maxima
     want_q, want_s = oracle.quant_e4m3_B(B16.astype(np.float32))
-    assert np.array_equal(ds.numpy().reshape(K // 16, chunks), want_s)
-    assert np.array_equal(got, want_q)
+    for call in range(3):  # repeated calls reuse the scratch
+        d8.fill_bytes(0xFF)
+        ds.fill_bytes(0)
+        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.ptr, work.ptr, None))
+        mm.synchronize()
+        q_k_minor = d8.numpy().reshape(K // 16, N, 16)
+        got = q_k_minor.transpose(0, 2, 1).reshape(K, N)
+        assert np.array_equal(ds.numpy(), want_s), f"call {call}"
+        assert np.array_equal(got, want_q), f"call {call}"
 
 
 @pytest.mark.parametrize("mode", ["bf16", "hilo", "fp8"])

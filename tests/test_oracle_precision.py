@@ -87,24 +87,69 @@ def test_group_quantiser_uses_each_groups_scale():
             assert np.all(np.abs(deq - x[g]) <= tol)
 
 
-def test_b_quantiser_slabs_and_chunks():
-    """One scale per (16-row slab, 256-column chunk); the last chunk ragged."""
+def _torch_e4m3(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.float8_e4m3fn).view(
+        torch.uint8).numpy()
+
+
+def _smallest_scale(amax):
+    """k minimal with amax / 2^k <= 448 (k >= -127), 0 for amax == 0."""
+    if amax == 0:
+        return 0
+    k = math.ceil(math.log2(amax / 448.0))
+    while amax / 2.0 ** (k - 1) <= 448.0:
+        k -= 1
+    while amax / 2.0 ** k > 448.0:
+        k += 1
+    return max(k, -127)
+
+
+def test_b_quantiser_column_chunks():
+    """One scale per 256-column chunk over all rows; the last chunk ragged."""
     rng = np.random.default_rng(5)
     K, N = 48, 600
     B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
     B[16:32, 256:512] *= 1e-3
     B[32:48, 512:] *= 50.0
     q, sc = oracle.quant_e4m3_B(B)
-    assert sc.shape == (3, 3)
-    for s in range(3):
-        for c in range(3):
-            blk = B[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)]
-            k = int(sc[s, c]) - 127
-            amax = float(np.abs(blk).max())
-            assert amax / 2.0**k <= 448.0 < amax / 2.0 ** (k - 1)
-            want = torch.from_numpy((blk / np.float32(2.0**k)).astype(np.float32)).to(
-                torch.float8_e4m3fn).view(torch.uint8).numpy()
-            assert np.array_equal(q[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)], want)
+    assert sc.shape == (3,)
+    for c in range(3):
+        blk = B[:, 256 * c:min(256 * c + 256, N)]
+        k = int(sc[c]) - 127
+        assert k == _smallest_scale(float(np.abs(blk).max()))
+        want = _torch_e4m3(blk / np.float32(2.0**k))
+        assert np.array_equal(q[:, 256 * c:min(256 * c + 256, N)], want)
+
+
+@pytest.mark.parametrize("bh,bw", [(16, 16), (32, 16), (64, 16), (16, 32)])
+def test_a_tiles_pair_consecutive_device_blocks(bh, bw):
+    """FP8 A: per 16-row slice, device blocks (stored blocks cut into 16-column
+    pieces, in order) (0,1), (2,3), ... share the E8M0 scale of their joint
+    absmax; restated here with PyTorch's cast."""
+    rng = np.random.default_rng(bh + bw)
+    block_rows, block_cols = 5, 7
+    cnt = np.array([0, 1, 2, 3, 7])
+    hdr = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    idx = np.concatenate([np.sort(rng.choice(block_cols, c, replace=False)) for c in cnt]).astype(
+        np.int32)
+    mags = 10.0 ** rng.uniform(-3, 2, (hdr[-1], bh, bw))
+    data = (mags * rng.choice([-1, 1], mags.shape)).astype(np.float32)
+    got = oracle.fp8_tiles_dequant(hdr, idx, data, bh, bw)
+    want = np.zeros_like(data)
+    pieces = bw // 16
+    for R in range(block_rows):
+        dev = [(k, q) for k in range(hdr[R], hdr[R + 1]) for q in range(pieces)]
+        for t in range(bh // 16):
+            rows = slice(16 * t, 16 * t + 16)
+            for d in range(0, len(dev), 2):
+                members = dev[d:d + 2]
+                amax = max(float(np.abs(data[k, rows, 16 * q:16 * q + 16]).max()) for k, q in members)
+                sc = 2.0 ** _smallest_scale(amax)
+                for k, q in members:
+                    v = data[k, rows, 16 * q:16 * q + 16]
+                    codes = _torch_e4m3(v / np.float32(sc))
+                    want[k, rows, 16 * q:16 * q + 16] = oracle.e4m3_to_f32(codes) * np.float32(sc)
+    assert np.array_equal(got, want)
 
 
 def test_bf16_matches_torch():
