@@ -225,6 +225,30 @@ int sparseMatrixMulTensorAdaptive64x16(const int *hdr, const int *idx, const voi
                                        const void *B, float *C, unsigned int M, unsigned int N,
                                        unsigned int colBlocksPerRow, void *stream);
 
+/* ---- CSR, hybrid and dense comparison ----------------------------------------
+ * bcsr_csr_spmm: C (M x N fp32) = A_csr . B, or C += A_csr . B when
+ * accumulate != 0; hdr[M + 1], idx[nnz] (column indices < K), data fp16,
+ * all device arrays; fp32 accumulation in CSR order.  B, C 16-byte aligned,
+ * N % 16 == 0.  Replaces sparseMatrixMult1Co (src/cuda_kernels.cu:271). */
+int bcsr_csr_spmm(const int32_t *hdr, const int32_t *idx, const void *data_f16, const void *B, float *C,
+                  int32_t M, int32_t K, int32_t N, int32_t accumulate, void *stream);
+/* src/cuda_kernels.cu:271 (square n x n, overwrites C). */
+int sparseMatrixMult1Co(const int *hdr, const int *idx, const void *data, const void *B, float *C,
+                        unsigned int n, void *stream);
+/* C[i] += P[i], count floats; addMatrices (src/cuda_kernels.cu:357) takes n x n. */
+int bcsr_add_into(float *C, const float *P, int64_t count, void *stream);
+int addMatrices(float *C, const float *CPart, unsigned int n, void *stream);
+/* The hybrid (HCSR) product of the reference's legacy driver (main.cu:567-590):
+ * C = plan . B (the dense blocks) then C += csr . B (the sparse remainder),
+ * without a second C buffer. */
+int bcsr_hybrid_spmm(const bcsr_plan *plan, const int32_t *csr_hdr, const int32_t *csr_idx,
+                     const void *csr_data_f16, const void *B, float *C, int32_t N, void *stream);
+/* Dense comparison row: C (M x N fp32) = A (M x K fp16) . B (K x N fp16),
+ * row-major, rocBLAS gemm_ex with fp32 accumulation (the reference's cuBLAS
+ * rows, nvbench_main.cu:1078-1180). */
+int bcsr_dense_gemm_f16(const void *A, const void *B, float *C, int32_t M, int32_t K, int32_t N,
+                        void *stream);
+
 /* ---- device plumbing ---------------------------------------------------------
  * Thin wrappers over the HIP runtime this library is linked against, so that
  * callers without their own HIP bindings (tests, bench, scripting languages)
@@ -265,7 +289,8 @@ int mg_generate_f16(int32_t rows, int32_t cols, double sparsity, const char *pat
                     int32_t bw, int32_t band_halfwidth, double inblock_density,
                     int32_t value_dist, double lo, double hi, uint32_t seed, void *out);
 /* mg::generate_bcsr: the BCSR form (bh x bw blocks) of mg_generate_f16's
- * matrix for the block patterns, generated block by block.  hdr must hold
+ * matrix, any pattern, generated one band of block-rows at a time (no dense
+ * matrix).  Rejects rows % bh or cols % bw != 0.  hdr must hold
  * rows / bh + 1 entries; *idx and *data (nnzb and nnzb * bh * bw floats) are
  * allocated by the library and released with mg_free. */
 int mg_generate_bcsr(int32_t rows, int32_t cols, double sparsity, const char *pattern, int32_t bh,

@@ -56,12 +56,10 @@ void generate_into(float *out, int rows, int cols, const GenSpec &spec, unsigned
 void generate_half_into(half *out, int rows, int cols, const GenSpec &spec, unsigned int seed);
 
 // The BCSR arrays (bh x bw blocks, float values) that BCSRMatrix would build
-// from generate_half_into(rows, cols, spec, seed), produced block by block
-// without a dense matrix: same draws, a block stored iff one of its values is
-// nonzero as a half.  For the block patterns whose blocks are the BCSR blocks:
-// "blockrandom", "blockrandom_rho", "blockband" (spec.bh x spec.bw == bh x bw)
-// and "pattern32by16" / "pattern64by16" (bh x 16).  Throws
-// std::invalid_argument otherwise or when rows % bh or cols % bw != 0.
+// from generate_half_into(rows, cols, spec, seed), produced one band of rows
+// at a time without a dense matrix: same draws, a block stored iff one of its
+// values is nonzero as a half.  Any pattern; throws std::invalid_argument for
+// an unknown pattern or when rows % bh or cols % bw != 0.
 struct BcsrArrays {
     std::vector<int32_t> hdr, idx;
     std::vector<float> data;

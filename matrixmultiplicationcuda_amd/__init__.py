@@ -146,6 +146,12 @@ def _declare(lib):
       ctypes.POINTER(ctypes.c_float), _P)
     f("bcsr_cm_train", i, _P, _P, _P, _P, _I32, ctypes.c_float, _I32, ctypes.c_float,
       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _P)
+    f("bcsr_csr_spmm", i, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P)
+    f("sparseMatrixMult1Co", i, _P, _P, _P, _P, _P, ctypes.c_uint, _P)
+    f("bcsr_add_into", i, _P, _P, ctypes.c_int64, _P)
+    f("addMatrices", i, _P, _P, ctypes.c_uint, _P)
+    f("bcsr_hybrid_spmm", i, _P, _P, _P, _P, _P, _P, _I32, _P)
+    f("bcsr_dense_gemm_f16", i, _P, _P, _P, _I32, _I32, _I32, _P)
     f("bcsr_copy_to_device", i, _P, _P, _P, _I32, _I32, _I32, _I32, ctypes.POINTER(_P),
       ctypes.POINTER(_P), ctypes.POINTER(_P))
     f("bcsr_release_device", i, _P)
@@ -264,8 +270,8 @@ class HostBCSR:
 
 def generate_bcsr(rows, cols, sparsity, pattern, bh=16, bw=16, seed=123, band_halfwidth=8,
                   inblock_density=1.0, value_dist="uniform", lo=1e-3, hi=1e2) -> HostBCSR:
-    """BCSR (float32 block values) of generate(...) for block patterns, built
-    block by block without the dense matrix (mg::generate_bcsr)."""
+    """BCSR (float32 block values) of generate(...), any pattern, built one
+    band of block-rows at a time without the dense matrix (mg::generate_bcsr)."""
     vd = {"uniform": 0, "loguniform": 1}[value_dist]
     hdr = np.zeros(rows // bh + 1, dtype=np.int32)
     nnzb = ctypes.c_int64(0)

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <numeric>
 #include <stdexcept>
 #include <type_traits>
 
@@ -154,51 +155,70 @@ void generate_half_into(half *out, int rows, int cols, const GenSpec &spec, unsi
 }
 
 void generate_bcsr(int rows, int cols, const GenSpec &spec, unsigned int seed, int bh, int bw, BcsrArrays &out) {
-    const std::string &p = spec.pattern;
-    bool aligned = false;
-    if (p == "blockrandom" || p == "blockrandom_rho" || p == "blockband") aligned = spec.bh == bh && spec.bw == bw;
-    if (p == "pattern32by16") aligned = bh == 32 && bw == 16;
-    if (p == "pattern64by16") aligned = bh == 64 && bw == 16;
-    if (!aligned) throw std::invalid_argument("generate_bcsr: pattern " + p + " does not generate " + std::to_string(bh) +
-                                              "x" + std::to_string(bw) + " blocks");
     if (bh <= 0 || bw <= 0 || rows % bh || cols % bw)
         throw std::invalid_argument("generate_bcsr: matrix is not a whole number of blocks");
-    const int nbr = rows / bh;
+    // Rows of one pattern block: every pattern visits its row bands in order
+    // (a band = the rows of one pattern block-row), so once a value lands in
+    // a later band the earlier ones are final.  The band buffer spans
+    // lcm(pattern rows, bh) rows so that it always holds whole BCSR block-rows.
+    const std::string &p = spec.pattern;
+    int prow = 1;
+    if (p == "blockrandom" || p == "blockrandom_rho" || p == "blockband" || p == "checkerboard" ||
+        p == "blockdiagonal")
+        prow = spec.bh;
+    else if (p == "pattern64by16")
+        prow = 64;
+    else if (p == "pattern32by16")
+        prow = 32;
+    else if (p != "random" && p != "diagonal")
+        throw std::invalid_argument("Unknown pattern: " + p);
+    if (prow <= 0) throw std::invalid_argument("generate_bcsr: bad pattern block size");
+    const int64_t g = std::gcd(static_cast<int64_t>(prow), static_cast<int64_t>(bh));
+    const int64_t band64 = static_cast<int64_t>(prow) / g * bh;
+    if (band64 > (int64_t(1) << 24)) throw std::invalid_argument("generate_bcsr: pattern band too tall");
+    const int band = static_cast<int>(std::min<int64_t>(band64, std::max(rows, 1)));
+    const int nbr = rows / bh, nbc = cols / bw;
     const size_t blk = static_cast<size_t>(bh) * bw;
     out.hdr.assign(nbr + 1, 0);
     out.idx.clear();
     out.data.clear();
-    // Blocks are drawn one after the other in (block-row, block-column) order,
-    // so values arrive block-contiguously: collect one block, then keep it if
-    // any value is nonzero as a half.
-    int cur_bi = -1, cur_bj = -1;
-    std::vector<float> cur(blk, 0.0f);
+    std::vector<float> buf(static_cast<size_t>(band) * cols, 0.0f);
+    int band_row0 = 0;  // first row held by buf
     auto flush = [&]() {
-        if (cur_bi < 0) return;
-        bool nz = false;
-        for (float v : cur)
-            if (float2half_rn(v) != static_cast<half>(0.0f)) {
-                nz = true;
-                break;
+        const int r1 = std::min(band_row0 + band, rows);
+        for (int bi = band_row0 / bh; bi < r1 / bh; ++bi)
+            for (int bj = 0; bj < nbc; ++bj) {
+                bool nz = false;
+                for (int i = 0; i < bh && !nz; ++i)
+                    for (int j = 0; j < bw; ++j)
+                        if (float2half_rn(buf[static_cast<size_t>(bi * bh + i - band_row0) * cols + bj * bw + j]) !=
+                            static_cast<half>(0.0f)) {
+                            nz = true;
+                            break;
+                        }
+                if (!nz) continue;
+                out.idx.push_back(bj);
+                for (int i = 0; i < bh; ++i) {
+                    const float *src = &buf[static_cast<size_t>(bi * bh + i - band_row0) * cols + bj * bw];
+                    out.data.insert(out.data.end(), src, src + bw);
+                }
+                out.hdr[bi + 1]++;
             }
-        if (nz) {
-            out.idx.push_back(cur_bj);
-            out.data.insert(out.data.end(), cur.begin(), cur.end());
-            out.hdr[cur_bi + 1]++;
-        }
-        std::fill(cur.begin(), cur.end(), 0.0f);
+        std::fill(buf.begin(), buf.end(), 0.0f);
     };
     generate_core<float>(rows, cols, spec, seed, [&](int i, int j, float v) {
-        const int bi = i / bh, bj = j / bw;
-        if (bi != cur_bi || bj != cur_bj) {
+        while (i >= band_row0 + band) {
             flush();
-            cur_bi = bi;
-            cur_bj = bj;
+            band_row0 += band;
         }
-        cur[static_cast<size_t>(i % bh) * bw + (j % bw)] = v;
+        buf[static_cast<size_t>(i - band_row0) * cols + j] = v;
     });
-    flush();
+    while (band_row0 < rows) {
+        flush();
+        band_row0 += band;
+    }
     for (int r = 0; r < nbr; ++r) out.hdr[r + 1] += out.hdr[r];
+    (void)blk;
 }
 
 template Matrix<float> generate_matrix<float>(int, int, double, const std::string &, int, unsigned int);

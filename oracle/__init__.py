@@ -67,6 +67,7 @@ def lib():
         f("orc_quant_e4m3_groups", None, _P, ctypes.c_int64, ctypes.c_int64, _P, _P)
         f("orc_quant_e4m3_B", None, _P, i, i, _P, _P)
         f("orc_quant_e4m3_tiles", None, _P, _P, _P, i, i, i, _P)
+        f("orc_csr_spmm_f32", None, _P, _P, _P, i, _P, i, _P)
         f("orc_f2bf16", ctypes.c_uint16, ctypes.c_float)
         f("orc_bf162f", ctypes.c_float, ctypes.c_uint16)
         f("orc_hilo_split", None, ctypes.c_float, ctypes.POINTER(ctypes.c_uint16),
@@ -282,6 +283,19 @@ def quant_e4m3_B(B):
     sc = np.empty(((N + 255) // 256,), dtype=np.uint8)
     lib().orc_quant_e4m3_B(_p(B), K, N, _p(q), _p(sc))
     return q, sc
+
+
+def csr_spmm_f32(hdr, idx, data16, Bf32):
+    """CSR x dense in CSR order (fp32, k ascending per row)."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.int32)
+    M = len(hdr) - 1
+    Bf = np.ascontiguousarray(Bf32, dtype=np.float32)
+    N = Bf.shape[1]
+    C = np.zeros((M, N), dtype=np.float32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
+    data = np.ascontiguousarray(data16, dtype=np.uint16) if len(data16) else np.zeros(1, np.uint16)
+    lib().orc_csr_spmm_f32(_p(hdr), _p(idx), _p(data), M, _p(Bf), N, _p(C))
+    return C
 
 
 def fp8_tiles_dequant(hdr, idx, data32, bh, bw):

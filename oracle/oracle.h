@@ -117,6 +117,12 @@ int64_t orc_csr_count(const uint16_t *A, int rows, int cols, int32_t *hdr);
 void orc_csr_fill(const uint16_t *A, int rows, int cols, const int32_t *hdr,
                   int32_t *idx, uint16_t *data);
 
+/* CSR x dense in CSR order, fp32 from +0 (the product sparseMatrixMult1Co
+ * computes, src/cuda_kernels.cu:271-285, restated on the CPU); pinned by
+ * exact equality with the dense cpu_matmul_ref loop on the densified CSR
+ * matrix (fp16 products are exact in fp32, zero terms add +0, k ascends). */
+void orc_csr_spmm_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf, int N,
+                      float *C);
 /* cpu_matmul_ref (dense, literal loop order). */
 void orc_matmul_dense_ref(const uint16_t *A, const uint16_t *B, int M, int K, int N,
                           float *C);

@@ -102,7 +102,9 @@ def test_bcsr_equals_oracle(bh, bw):
 @pytest.mark.parametrize("pattern,bh,gbh,kw", [
     ("blockrandom", 16, 16, {}), ("blockrandom", 32, 32, {"value_dist": "loguniform"}),
     ("blockrandom_rho", 16, 16, {"inblock_density": 0.3}), ("blockband", 16, 16, {"band_halfwidth": 2}),
-    ("pattern32by16", 32, 16, {}), ("pattern64by16", 64, 16, {})])
+    ("pattern32by16", 32, 16, {}), ("pattern64by16", 64, 16, {}), ("pattern64by16", 32, 16, {}),
+    ("pattern32by16", 64, 16, {}), ("random", 16, 16, {}), ("diagonal", 16, 16, {}),
+    ("checkerboard", 32, 32, {}), ("blockdiagonal", 16, 16, {})])
 def test_generate_bcsr_equals_dense_path(pattern, bh, gbh, kw):
     """The block-by-block generator == the oracle's dense generator followed by
     the oracle's BCSR builder (same blocks, same fp32 values)."""
@@ -117,10 +119,7 @@ def test_generate_bcsr_equals_dense_path(pattern, bh, gbh, kw):
 
 def test_generate_bcsr_rejects_unaligned():
     with pytest.raises(mm.BcsrError) as e:
-        mm.generate_bcsr(64, 64, 0.5, "random", bh=16, bw=16)
-    assert e.value.status == -8
-    with pytest.raises(mm.BcsrError) as e:
-        mm.generate_bcsr(64, 64, 0.5, "pattern64by16", bh=32, bw=16)  # its blocks are 64 x 16
+        mm.generate_bcsr(64, 64, 0.5, "nosuchpattern", bh=16, bw=16)
     assert e.value.status == -8
     with pytest.raises(mm.BcsrError) as e:
         mm.generate_bcsr(72, 64, 0.5, "blockrandom", bh=16, bw=16)

@@ -185,6 +185,16 @@ def test_csr_matches_scipy():
     assert np.array_equal(oracle.f16_bits_to_f32(data), S.data)
 
 
+def test_csr_product_equals_dense_cpu_matmul_ref_bitwise():
+    """CSR-order product == the literal dense cpu_matmul_ref loop, bit for bit."""
+    A = oracle.generate_matrix(80, 112, 0.85, "random", 16, 4).astype(np.float32)
+    B = oracle.generate_matrix(112, 48, 0.0, "random", 16, 5).astype(np.float32)
+    A16, B16 = oracle.f32_to_f16_bits(A), oracle.f32_to_f16_bits(B)
+    hdr, idx, data = oracle.csr_from_dense(A16)
+    C = oracle.csr_spmm_f32(hdr, idx, data, oracle.f16_bits_to_f32(B16))
+    assert np.array_equal(C, oracle.matmul_dense_ref(A16, B16))
+
+
 # ---- products ------------------------------------------------------------------------
 def small_case(M=256, K=384, N=96, sp=0.8, seed=9):
     A = oracle.generate_matrix(M, K, sp, "blockrandom", 16, seed).astype(np.float32)
