@@ -11,6 +11,7 @@ dominant kernel's roofline position (algorithmic bytes / time vs 8 TB/s
 HBM) and the oracle's CPU product timed on this host as cpu_baseline.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -93,6 +94,55 @@ def precision_modes(mm, np, h16, h32, B, N, stream, ref_ms):
                      "vs_f16": round(ref_ms / ms, 3), "kernel": plan.kernel_name(N)}
         C.free()
         plan.close()
+    return out
+
+
+def library_rows(mm, np, h, B, N, stream, ref_ms, C_ref):
+    """The library comparison rows at the target workload: rocBLAS's dense fp16
+    GEMM on the densified A (the reference's cuBLAS rows) and rocSPARSE's BSR
+    SpMM on the plain BCSR arrays, each timed like the kernel above and
+    checked against its output."""
+    M, K = h.block_rows * h.bh, h.block_cols * h.bw
+    flops_sparse = 2.0 * h.nnzb * h.bh * h.bw * N
+    C = mm.DeviceArray((M, N), np.float32)
+    out = {}
+
+    def check(name):
+        got = C.numpy().astype(np.float64)
+        ref = C_ref.astype(np.float64)
+        out[name]["max_abs_diff_vs_bcsr"] = float(np.abs(got - ref).max(initial=0.0))
+
+    # dense: A densified on the host (M x K fp16)
+    A = np.zeros((h.block_rows, h.bh, h.block_cols, h.bw), np.float16)
+    rows = np.repeat(np.arange(h.block_rows), np.diff(h.hdr))
+    A[rows, :, h.idx, :] = h.data.astype(np.float16)
+    dA = mm.DeviceArray.from_numpy(A.reshape(M, K))
+    del A
+    fn = lambda: mm.check(mm.lib().bcsr_dense_gemm_f16(dA.ptr, B.ptr, C.ptr, M, K, N, stream.handle))
+    ms = time_ms(stream, fn, warmup=3, reps=20)
+    out["rocblas_dense_f16"] = {"ms": round(ms, 5), "dense_tflops": round(2.0 * M * K * N / ms / 1e9, 2),
+                                "effective_tflops": round(flops_sparse / ms / 1e9, 2),
+                                "bcsr_speedup": round(ms / ref_ms, 3)}
+    check("rocblas_dense_f16")
+    dA.free()
+    # rocSPARSE BSR (square blocks only)
+    if h.bh == h.bw:
+        dh = mm.DeviceArray.from_numpy(np.ascontiguousarray(h.hdr, np.int32))
+        di = mm.DeviceArray.from_numpy(np.ascontiguousarray(h.idx, np.int32))
+        dd = mm.DeviceArray.from_numpy(np.ascontiguousarray(h.data, np.float16))
+        ref = ctypes.c_void_p()
+        mm.check(mm.lib().bcsr_rocsparse_create(dh.ptr, di.ptr, dd.ptr, h.block_rows, h.block_cols,
+                                                h.nnzb, h.bh, B.ptr, C.ptr, N, ctypes.byref(ref)),
+                 "rocsparse bsr")
+        fn = lambda: mm.check(mm.lib().bcsr_rocsparse_run(ref, stream.handle))
+        ms = time_ms(stream, fn, warmup=3, reps=20)
+        out["rocsparse_bsr_f16"] = {"ms": round(ms, 5), "effective_tflops": round(flops_sparse / ms / 1e9, 2),
+                                    "bcsr_speedup": round(ms / ref_ms, 3)}
+        check("rocsparse_bsr_f16")
+        mm.lib().bcsr_rocsparse_destroy(ref)
+        for a in (dh, di, dd):
+            a.free()
+    C.free()
     return out
 
 
@@ -305,6 +355,7 @@ def main():
         t1 = time.time()
         extensions = {"precision_modes_at_this_workload":
                       precision_modes(mm, np, h, h32, B, N, stream, ms)}
+        extensions["library_rows_at_this_workload"] = library_rows(mm, np, h, B, N, stream, ms, C_gpu)
         extensions["batched"] = batched(mm, np, args.batch_count, stream)
         print(f"# extensions {time.time() - t1:.1f}s", file=sys.stderr, flush=True)
 

@@ -226,15 +226,20 @@ int sparseMatrixMulTensorAdaptive64x16(const int *hdr, const int *idx, const voi
                                        unsigned int colBlocksPerRow, void *stream);
 
 /* ---- CSR, hybrid and dense comparison ----------------------------------------
- * bcsr_csr_spmm: C (M x N fp32) = A_csr . B, or C += A_csr . B when
- * accumulate != 0; hdr[M + 1], idx[nnz] (column indices < K), data fp16,
- * all device arrays; fp32 accumulation in CSR order.  B, C 16-byte aligned,
- * N % 16 == 0.  Replaces sparseMatrixMult1Co (src/cuda_kernels.cu:271). */
+ * bcsr_csr_spmm: C (M x N fp32) = A_csr . B (accumulate 0), C = C + (A_csr . B)
+ * (accumulate 1: the finished row sum added to C) or the running sum started
+ * from C (accumulate 2: ((C + a0 b0) + a1 b1) + ...); hdr[M + 1], idx[nnz]
+ * (column indices < K), data fp16, all device arrays; fp32 accumulation in
+ * CSR order.  B, C 16-byte aligned, N % 16 == 0.  Replaces sparseMatrixMult1Co
+ * and sparseMatrixMult1 (src/cuda_kernels.cu:271-299). */
 int bcsr_csr_spmm(const int32_t *hdr, const int32_t *idx, const void *data_f16, const void *B, float *C,
                   int32_t M, int32_t K, int32_t N, int32_t accumulate, void *stream);
 /* src/cuda_kernels.cu:271 (square n x n, overwrites C). */
 int sparseMatrixMult1Co(const int *hdr, const int *idx, const void *data, const void *B, float *C,
                         unsigned int n, void *stream);
+/* src/cuda_kernels.cu:287 (square n x n, C += A . B in place, summed from C). */
+int sparseMatrixMult1(const int *hdr, const int *idx, const void *data, const void *B, float *C,
+                      unsigned int n, void *stream);
 /* C[i] += P[i], count floats; addMatrices (src/cuda_kernels.cu:357) takes n x n. */
 int bcsr_add_into(float *C, const float *P, int64_t count, void *stream);
 int addMatrices(float *C, const float *CPart, unsigned int n, void *stream);
@@ -248,6 +253,24 @@ int bcsr_hybrid_spmm(const bcsr_plan *plan, const int32_t *csr_hdr, const int32_
  * rows, nvbench_main.cu:1078-1180). */
 int bcsr_dense_gemm_f16(const void *A, const void *B, float *C, int32_t M, int32_t K, int32_t N,
                         void *stream);
+
+/* Library comparison rows: rocSPARSE generic SpMM, fp32 compute, B (K x N
+ * fp16 row-major) into C (M x N fp32 row-major), on the plain BCSR arrays
+ * (device hdr[block_rows + 1], idx[nnzb], data: nnzb square block_dim blocks,
+ * each row-major, fp16) or on CSR arrays (hdr[M + 1], idx[nnz], data[nnz]).
+ * The reference's counterpart is its cuSPARSE CSR row (main.cu:530-563).
+ * create does the descriptors, the work buffer and the analysis (blocking);
+ * run computes on a stream. */
+typedef struct bcsr_sparse_ref bcsr_sparse_ref;
+int bcsr_rocsparse_create(const int32_t *hdr, const int32_t *idx, const void *data_f16,
+                          int32_t block_rows, int32_t block_cols, int64_t nnzb,
+                          int32_t block_dim, const void *B, float *C, int32_t N,
+                          bcsr_sparse_ref **out);
+int bcsr_rocsparse_csr_create(const int32_t *hdr, const int32_t *idx, const void *data_f16,
+                              int32_t M, int32_t K, int64_t nnz, const void *B, float *C,
+                              int32_t N, bcsr_sparse_ref **out);
+int bcsr_rocsparse_run(bcsr_sparse_ref *ref, void *stream);
+int bcsr_rocsparse_destroy(bcsr_sparse_ref *ref);
 
 /* ---- device plumbing ---------------------------------------------------------
  * Thin wrappers over the HIP runtime this library is linked against, so that

@@ -148,10 +148,17 @@ def _declare(lib):
       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float), _P)
     f("bcsr_csr_spmm", i, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P)
     f("sparseMatrixMult1Co", i, _P, _P, _P, _P, _P, ctypes.c_uint, _P)
+    f("sparseMatrixMult1", i, _P, _P, _P, _P, _P, ctypes.c_uint, _P)
     f("bcsr_add_into", i, _P, _P, ctypes.c_int64, _P)
     f("addMatrices", i, _P, _P, ctypes.c_uint, _P)
     f("bcsr_hybrid_spmm", i, _P, _P, _P, _P, _P, _P, _I32, _P)
     f("bcsr_dense_gemm_f16", i, _P, _P, _P, _I32, _I32, _I32, _P)
+    f("bcsr_rocsparse_create", i, _P, _P, _P, _I32, _I32, ctypes.c_int64, _I32, _P, _P, _I32,
+      ctypes.POINTER(ctypes.c_void_p))
+    f("bcsr_rocsparse_csr_create", i, _P, _P, _P, _I32, _I32, ctypes.c_int64, _P, _P, _I32,
+      ctypes.POINTER(ctypes.c_void_p))
+    f("bcsr_rocsparse_run", i, _P, _P)
+    f("bcsr_rocsparse_destroy", i, _P)
     f("bcsr_copy_to_device", i, _P, _P, _P, _I32, _I32, _I32, _I32, ctypes.POINTER(_P),
       ctypes.POINTER(_P), ctypes.POINTER(_P))
     f("bcsr_release_device", i, _P)

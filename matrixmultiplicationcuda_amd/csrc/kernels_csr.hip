@@ -6,9 +6,12 @@
 // one wave per (row, 256-column tile); lane l owns columns tile + 4 l .. + 3
 // and walks the row's nonzeros in CSR order, loading 8 bytes of B per
 // nonzero and accumulating in fp32 (products of two fp16 values are exact in
-// fp32, so the result is the CPU k-order sum bit for bit).  With accumulate
-// set it adds to C instead of overwriting it, which is how the hybrid path
-// adds its CSR part onto the BCSR part without a second buffer.
+// fp32, so the result is the CPU k-order sum bit for bit).  accumulate = 1
+// adds the finished row sum to C (C + sum, the order of the reference's
+// addMatrices after sparseMatrixMult1Co), which is how the hybrid path adds
+// its CSR part onto the BCSR part without a second buffer; accumulate = 2
+// starts the running sum from C (((C + a0 b0) + a1 b1) + ..., the order of
+// the reference's in-place sparseMatrixMult1, src/cuda_kernels.cu:287-299).
 //
 // Dense GEMM: rocBLAS gemm_ex (fp16 inputs, fp32 output and accumulation),
 // the comparison row the reference takes from cuBLAS (nvbench_main.cu:
@@ -35,7 +38,15 @@ __global__ __launch_bounds__(256) void csr_spmm(const int *__restrict__ hdr, con
     const int n0 = tile * kCsrTile + 4 * lane;
     const bool valid = n0 < N;  // N % 16 == 0: all four columns or none
     const int p0 = hdr[row], p1 = hdr[row + 1];
+    f32x4 *cp = reinterpret_cast<f32x4 *>(C + static_cast<size_t>(row) * N + (valid ? n0 : 0));
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (accumulate == 2 && valid) {
+        const f32x4 c = *cp;
+        acc[0] = c[0];
+        acc[1] = c[1];
+        acc[2] = c[2];
+        acc[3] = c[3];
+    }
     const _Float16 *bcol = B + (valid ? n0 : 0);
     for (int p = p0; p < p1; ++p) {
         const int k = idx[p];
@@ -51,9 +62,8 @@ __global__ __launch_bounds__(256) void csr_spmm(const int *__restrict__ hdr, con
         acc[3] = fmaf(a, b3, acc[3]);
     }
     if (!valid) return;
-    f32x4 *cp = reinterpret_cast<f32x4 *>(C + static_cast<size_t>(row) * N + n0);
     f32x4 out{acc[0], acc[1], acc[2], acc[3]};
-    if (accumulate) out += *cp;
+    if (accumulate == 1) out += *cp;
     *cp = out;
 }
 
@@ -94,7 +104,7 @@ int blas_handle(rocblas_handle &h) {
 
 extern "C" int bcsr_csr_spmm(const int32_t *hdr, const int32_t *idx, const void *data_f16, const void *B, float *C,
                              int32_t M, int32_t K, int32_t N, int32_t accumulate, void *stream) {
-    if (M < 0 || K < 0) return BCSR_ERR_INVALID_ARG;
+    if (M < 0 || K < 0 || accumulate < 0 || accumulate > 2) return BCSR_ERR_INVALID_ARG;
     int rc = check_dense_operands(B, C, N);
     if (rc != BCSR_OK) return rc;
     if (!M || !N) return BCSR_OK;
@@ -105,8 +115,15 @@ extern "C" int bcsr_csr_spmm(const int32_t *hdr, const int32_t *idx, const void 
     const unsigned grid = static_cast<unsigned>((waves + 3) / 4);
     hipLaunchKernelGGL(csr_spmm, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), hdr, idx,
                        static_cast<const _Float16 *>(data_f16), static_cast<const _Float16 *>(B), C, M, N, n_tiles,
-                       accumulate ? 1 : 0);
+                       accumulate);
     return hip_status(hipGetLastError());
+}
+
+extern "C" int sparseMatrixMult1(const int *hdr, const int *idx, const void *data, const void *B, float *C,
+                                 unsigned int n, void *stream) {
+    if (n > static_cast<unsigned>(INT32_MAX)) return BCSR_ERR_INVALID_ARG;
+    return bcsr_csr_spmm(hdr, idx, data, B, C, static_cast<int32_t>(n), static_cast<int32_t>(n),
+                         static_cast<int32_t>(n), 2, stream);
 }
 
 extern "C" int bcsr_add_into(float *C, const float *P, int64_t count, void *stream) {

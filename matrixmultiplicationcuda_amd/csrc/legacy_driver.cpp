@@ -4,7 +4,8 @@
 //     name,time,diff,maxrelativeerr,size,pattern,sparsity,matrixApath
 // (main.cu:81-82): the median of the 20 timed runs in milliseconds, then
 // maxdiff and avgrelerr (miscutil.cpp:47-74) against the dense product, which
-// is the correctness reference as in the reference's driver (main.cu:427-428).
+// is the correctness reference as in the reference's driver (main.cu:427-428);
+// here that product is rocBLAS's fp16 GEMM with fp32 accumulation.
 //
 // Usage: bcsr_legacy_driver A.mat B.mat [pattern] [sparsity]
 // Unlike the reference it accepts rectangular A (M x K) and B (K x N) with
@@ -137,7 +138,7 @@ int main(int argc, char **argv) {
         cx.host_c.resize(mn);
 
         // dense product: the correctness reference of the remaining rows
-        run20(cx, "Dense rocBLAS", [&] {
+        run20(cx, "rocBLAS GeMM", [&] {
             bcsr_check(bcsr_dense_gemm_f16(dA.p, dB.p, cx.gpuC, cx.M, cx.K, cx.N, nullptr), "dense");
         }, true);
 
@@ -162,6 +163,26 @@ int main(int argc, char **argv) {
             run20(cx, "SpMM with Tensors Op", [&] {
                 bcsr_check(sparseMatrixMulTensor1(bh, bi, bd, dB.p, cx.gpuC, cx.N, nullptr), "spmm1");
             });
+
+        // vendor sparse library rows (the reference's cuSPARSE CSR, main.cu:530-563)
+        {
+            bcsr_sparse_ref *ref = nullptr;
+            bcsr_check(bcsr_rocsparse_csr_create(ch, ci, cd, cx.M, cx.K, csr.nnz(), dB.p, cx.gpuC, cx.N, &ref),
+                       "rocsparse csr");
+            run20(cx, "rocSPARSE CSR", [&] { bcsr_check(bcsr_rocsparse_run(ref, nullptr), "rocsparse csr run"); });
+            bcsr_rocsparse_destroy(ref);
+            const size_t nnzb = static_cast<size_t>(bcsr.nnzb());
+            DevBuf ph(sizeof(int) * (bcsr.blockRows + 1)), pi(sizeof(int) * (nnzb ? nnzb : 1)),
+                pd(sizeof(half) * (nnzb ? nnzb : 1) * 256);
+            hip_check(hipMemcpy(ph.p, bcsr.hdr, sizeof(int) * (bcsr.blockRows + 1), hipMemcpyHostToDevice), "hdr");
+            hip_check(hipMemcpy(pi.p, bcsr.idx, sizeof(int) * nnzb, hipMemcpyHostToDevice), "idx");
+            hip_check(hipMemcpy(pd.p, bcsr.data, sizeof(half) * nnzb * 256, hipMemcpyHostToDevice), "data");
+            bcsr_check(bcsr_rocsparse_create(static_cast<int *>(ph.p), static_cast<int *>(pi.p), pd.p, bcsr.blockRows,
+                                             bcsr.blockCols, static_cast<int64_t>(nnzb), 16, dB.p, cx.gpuC, cx.N, &ref),
+                       "rocsparse bsr");
+            run20(cx, "rocSPARSE BSR", [&] { bcsr_check(bcsr_rocsparse_run(ref, nullptr), "rocsparse bsr run"); });
+            bcsr_rocsparse_destroy(ref);
+        }
 
         // storage-precision modes of this library
         const std::vector<float> f32 = blocks_f32(bcsr);

@@ -68,6 +68,7 @@ def lib():
         f("orc_quant_e4m3_B", None, _P, i, i, _P, _P)
         f("orc_quant_e4m3_tiles", None, _P, _P, _P, i, i, i, _P)
         f("orc_csr_spmm_f32", None, _P, _P, _P, i, _P, i, _P)
+        f("orc_csr_spmm_acc_f32", None, _P, _P, _P, i, _P, i, _P)
         f("orc_f2bf16", ctypes.c_uint16, ctypes.c_float)
         f("orc_bf162f", ctypes.c_float, ctypes.c_uint16)
         f("orc_hilo_split", None, ctypes.c_float, ctypes.POINTER(ctypes.c_uint16),
@@ -285,16 +286,21 @@ def quant_e4m3_B(B):
     return q, sc
 
 
-def csr_spmm_f32(hdr, idx, data16, Bf32):
-    """CSR x dense in CSR order (fp32, k ascending per row)."""
+def csr_spmm_f32(hdr, idx, data16, Bf32, C0=None):
+    """CSR x dense in CSR order (fp32, k ascending per row), from +0 or, with
+    C0, as the running sum started from C0 (sparseMatrixMult1's order)."""
     hdr = np.ascontiguousarray(hdr, dtype=np.int32)
     M = len(hdr) - 1
     Bf = np.ascontiguousarray(Bf32, dtype=np.float32)
     N = Bf.shape[1]
-    C = np.zeros((M, N), dtype=np.float32)
     idx = np.ascontiguousarray(idx, dtype=np.int32) if len(idx) else np.zeros(1, np.int32)
     data = np.ascontiguousarray(data16, dtype=np.uint16) if len(data16) else np.zeros(1, np.uint16)
-    lib().orc_csr_spmm_f32(_p(hdr), _p(idx), _p(data), M, _p(Bf), N, _p(C))
+    if C0 is None:
+        C = np.zeros((M, N), dtype=np.float32)
+        lib().orc_csr_spmm_f32(_p(hdr), _p(idx), _p(data), M, _p(Bf), N, _p(C))
+    else:
+        C = np.array(C0, dtype=np.float32, order="C", copy=True).reshape(M, N)
+        lib().orc_csr_spmm_acc_f32(_p(hdr), _p(idx), _p(data), M, _p(Bf), N, _p(C))
     return C
 
 

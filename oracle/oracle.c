@@ -542,17 +542,22 @@ void orc_bcsr_spmm_f64(const int32_t *hdr, const int32_t *idx, const float *data
     orc_run_split(jb, nthreads, orc_spmm_f64_thread);
 }
 
-void orc_csr_spmm_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf, int N,
-                      float *C) {
+void orc_csr_spmm_acc_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf,
+                          int N, float *C) {
     for (int i = 0; i < M; ++i) {
         float *c = C + (size_t)i * N;
-        for (int j = 0; j < N; ++j) c[j] = 0.0f;
         for (int32_t p = hdr[i]; p < hdr[i + 1]; ++p) {
             const float a = orc_h2f(data[p]);
             const float *b = Bf + (size_t)idx[p] * N;
             for (int j = 0; j < N; ++j) c[j] += a * b[j];
         }
     }
+}
+
+void orc_csr_spmm_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf, int N,
+                      float *C) {
+    memset(C, 0, sizeof(float) * (size_t)M * (size_t)N);
+    orc_csr_spmm_acc_f32(hdr, idx, data, M, Bf, N, C);
 }
 
 /* ---------------------------------------------------------------------------

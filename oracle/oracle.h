@@ -123,6 +123,12 @@ void orc_csr_fill(const uint16_t *A, int rows, int cols, const int32_t *hdr,
  * matrix (fp16 products are exact in fp32, zero terms add +0, k ascends). */
 void orc_csr_spmm_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf, int N,
                       float *C);
+/* The same loop summing onto what C holds, the in-place running sum of
+ * sparseMatrixMult1 (src/cuda_kernels.cu:287-299): c = ((c + a0 b0) + a1 b1)...;
+ * pinned by C = 0 giving orc_csr_spmm_f32 and by a rounding case whose result
+ * differs from c + (a0 b0 + a1 b1). */
+void orc_csr_spmm_acc_f32(const int32_t *hdr, const int32_t *idx, const uint16_t *data, int M, const float *Bf,
+                          int N, float *C);
 /* cpu_matmul_ref (dense, literal loop order). */
 void orc_matmul_dense_ref(const uint16_t *A, const uint16_t *B, int M, int K, int N,
                           float *C);

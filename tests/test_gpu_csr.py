@@ -66,6 +66,13 @@ def test_reference_names_and_errors():
     mm.synchronize()
     want = oracle.csr_spmm_f32(hdr, idx, data, B16.astype(np.float32)) + np.float32(1.0)
     assert np.array_equal(dC.numpy(), want)
+    # sparseMatrixMult1: the running sum starts from what C holds
+    X = np.random.default_rng(2).standard_normal((n, n)).astype(np.float32) * np.float32(1e3)
+    dX = dev(X)
+    mm.check(mm.lib().sparseMatrixMult1(dh.ptr, di.ptr, dd.ptr, dB.ptr, dX.ptr, n, None))
+    mm.synchronize()
+    assert np.array_equal(dX.numpy(), oracle.csr_spmm_f32(hdr, idx, data, B16.astype(np.float32), C0=X))
+    assert mm.lib().bcsr_csr_spmm(dh.ptr, di.ptr, dd.ptr, dB.ptr, dC.ptr, n, n, 16, 3, None) == -1
     assert mm.lib().bcsr_csr_spmm(dh.ptr, di.ptr, dd.ptr, dB.ptr, dC.ptr, n, n, 40, 0, None) == -3
     assert mm.lib().bcsr_csr_spmm(dh.ptr, di.ptr, dd.ptr, dB.ptr, dC.ptr, -1, n, n, 0, None) == -1
 

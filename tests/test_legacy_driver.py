@@ -46,8 +46,8 @@ def test_driver_csv_rows(tmp_path):
     assert lines[0].startswith("Matrix A: 256 x 256, nonZeros: ")
     rows = [ln.split(",") for ln in lines[1:]]
     names = [x[0] for x in rows]
-    for want in ["Dense rocBLAS", "SpMM 1 Co", "SpMM with Tensors", "SpMM with Tensors Op",
-                 "SpMM BF16", "SpMM FP8 E4M3", "SpMM hi+lo"] + \
+    for want in ["rocBLAS GeMM", "SpMM 1 Co", "SpMM with Tensors", "SpMM with Tensors Op",
+                 "rocSPARSE CSR", "rocSPARSE BSR", "SpMM BF16", "SpMM FP8 E4M3", "SpMM hi+lo"] + \
             [f"Hybrid CSR {t:.6f}" for t in (0.0, 0.2, 0.4, 0.6, 0.8)]:
         assert want in names, want
     for x in rows:

@@ -195,6 +195,28 @@ def test_csr_product_equals_dense_cpu_matmul_ref_bitwise():
     assert np.array_equal(C, oracle.matmul_dense_ref(A16, B16))
 
 
+def test_csr_running_sum_from_c():
+    """sparseMatrixMult1's in-place order: C = 0 gives the plain product; a
+    row of two unit products onto 2^24 rounds each step back to 2^24 (ties to
+    even), where 2^24 + (1 + 1) would be exact."""
+    A = oracle.generate_matrix(48, 64, 0.8, "random", 16, 8).astype(np.float32)
+    B = oracle.generate_matrix(64, 32, 0.0, "random", 16, 9).astype(np.float32)
+    A16, B16 = oracle.f32_to_f16_bits(A), oracle.f32_to_f16_bits(B)
+    hdr, idx, data = oracle.csr_from_dense(A16)
+    Bf = oracle.f16_bits_to_f32(B16)
+    assert np.array_equal(oracle.csr_spmm_f32(hdr, idx, data, Bf, C0=np.zeros((48, 32), np.float32)),
+                          oracle.csr_spmm_f32(hdr, idx, data, Bf))
+    one = oracle.f32_to_f16_bits(np.array([1.0], np.float32))[0]
+    hdr1 = np.array([0, 2], np.int32)
+    idx1 = np.array([0, 1], np.int32)
+    data1 = np.array([one, one], np.uint16)
+    B1 = np.ones((2, 16), np.float32)
+    C0 = np.full((1, 16), 2.0**24, np.float32)
+    got = oracle.csr_spmm_f32(hdr1, idx1, data1, B1, C0=C0)
+    assert np.all(got == np.float32(2.0**24))
+    assert np.all(C0 + oracle.csr_spmm_f32(hdr1, idx1, data1, B1) == np.float32(2.0**24 + 2))
+
+
 # ---- products ------------------------------------------------------------------------
 def small_case(M=256, K=384, N=96, sp=0.8, seed=9):
     A = oracle.generate_matrix(M, K, sp, "blockrandom", 16, seed).astype(np.float32)
