@@ -115,8 +115,16 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, const
     __shared__ unsigned rec[2][kChunk * 4];
     SlabPart sp;
     load_slab_part(B, N, slabs, x, h, sp);
+    // the chunk's partials: the first 8 per thread as unconditional loads of
+    // a clamped index (all in flight at once), the rest (K > 16384) in a loop
     float am = 0.0f;
-    for (int k = threadIdx.x; k < slabs; k += 128) am = fmaxf(am, part[static_cast<size_t>(k) * chunks + h]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = threadIdx.x + 128 * i;
+        const float v = part[static_cast<size_t>(k < slabs ? k : slabs - 1) * chunks + h];
+        am = fmaxf(am, v);
+    }
+    for (int k = threadIdx.x + 1024; k < slabs; k += 128) am = fmaxf(am, part[static_cast<size_t>(k) * chunks + h]);
     am = block_max128(am, wmax);
     const unsigned se = e8m0_for_absmax(am);
     if (x == 0 && threadIdx.x == 0) scale[h] = static_cast<uint8_t>(se);
