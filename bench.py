@@ -146,6 +146,64 @@ def library_rows(mm, np, h, B, N, stream, ref_ms, C_ref):
     return out
 
 
+# The reference's published medians (ms) of its legacy "SpMM with Tensors"
+# kernel: square n x n A with 16 x 16 blocks ("blockrandom", fraction of zero
+# blocks as given), B n x n, N = n; column time_med of avg_results.csv:86-113
+# (hardware not recorded by the reference).
+PUBLISHED_SPMM_WITH_TENSORS_MS = {
+    5376: {0.7: 17.9727365, 0.8: 11.217856, 0.4: 36.245167, 0.9: 5.721632, 0.6: 26.1274395,
+           0.5: 31.784544, 0.3: 40.057983},
+    1280: {0.7: 0.192896, 0.8: 0.150608, 0.4: 0.314752, 0.9: 0.111984, 0.6: 0.235488,
+           0.5: 0.275552, 0.3: 0.356448},
+    4096: {0.7: 4.937264, 0.8: 3.390464, 0.4: 13.035104, 0.9: 1.917968, 0.6: 6.68472,
+           0.5: 9.60736, 0.3: 15.07904},
+    8192: {0.7: 80.295952, 0.8: 52.0944155, 0.4: 158.6035995, 0.9: 24.801424, 0.6: 105.1238405,
+           0.5: 130.032112, 0.3: 252.054352},
+}
+
+
+def published_shapes(mm, np, stream):
+    """This library's fp16 kernel on the reference's published shapes, each
+    checked against rocBLAS's dense product of the same operands."""
+    rows = []
+    for n, by_sp in sorted(PUBLISHED_SPMM_WITH_TENSORS_MS.items()):
+        B16 = mm.generate(n, n, 0.0, "random", bh=16, seed=456)
+        dB = mm.DeviceArray.from_numpy(B16)
+        del B16
+        C = mm.DeviceArray((n, n), np.float32)
+        Cd = mm.DeviceArray((n, n), np.float32)
+        for sp, ref_ms in sorted(by_sp.items()):
+            h32 = mm.generate_bcsr(n, n, sp, "blockrandom", bh=16, bw=16, seed=123)
+            h = mm.HostBCSR(h32.hdr, h32.idx, h32.data.astype(np.float16), h32.block_rows, h32.block_cols,
+                            16, 16)
+            del h32
+            plan = mm.Plan(h, "f16")
+            reps = 20 if n <= 4096 else 5
+            ms = time_ms(stream, lambda: plan.spmm_ptr(dB.ptr, C.ptr, n, stream), warmup=2, reps=reps)
+            A = np.zeros((h.block_rows, 16, h.block_cols, 16), np.float16)
+            A[np.repeat(np.arange(h.block_rows), np.diff(h.hdr)), :, h.idx, :] = h.data
+            dA = mm.DeviceArray.from_numpy(A.reshape(n, n))
+            del A
+            mm.check(mm.lib().bcsr_dense_gemm_f16(dA.ptr, dB.ptr, Cd.ptr, n, n, n, stream.handle))
+            stream.synchronize()
+            diff = float(np.abs(C.numpy() - Cd.numpy()).max(initial=0.0))
+            dA.free()
+            flops = 2.0 * h.nnzb * 256 * n
+            rows.append({"n": n, "zero_block_fraction": sp, "nnzb": int(h.nnzb), "ms": round(ms, 5),
+                         "effective_tflops": round(flops / ms / 1e9, 2),
+                         "dense_equivalent_tflops": round(2.0 * n ** 3 / ms / 1e9, 2),
+                         "reference_ms": ref_ms, "speedup_vs_reference": round(ref_ms / ms, 2),
+                         "max_abs_diff_vs_rocblas": diff, "kernel": plan.kernel_name(n)})
+            plan.close()
+        for a in (dB, C, Cd):
+            a.free()
+    sp_list = [r["speedup_vs_reference"] for r in rows]
+    return {"workload": "square n x n, 16 x 16 blockrandom blocks, N = n (the reference's published runs, "
+                        "avg_results.csv:86-113, its hardware not recorded)",
+            "geomean_speedup_vs_reference": round(float(np.exp(np.mean(np.log(sp_list)))), 2),
+            "min_speedup_vs_reference": min(sp_list), "rows": rows}
+
+
 def batched(mm, np, count, stream):
     """The batched uneven workload: plain fp16 per-matrix launches (the
     library's static choice each) against the batched persistent kernel
@@ -356,6 +414,7 @@ def main():
         extensions = {"precision_modes_at_this_workload":
                       precision_modes(mm, np, h, h32, B, N, stream, ms)}
         extensions["library_rows_at_this_workload"] = library_rows(mm, np, h, B, N, stream, ms, C_gpu)
+        extensions["reference_published_shapes"] = published_shapes(mm, np, stream)
         extensions["batched"] = batched(mm, np, args.batch_count, stream)
         print(f"# extensions {time.time() - t1:.1f}s", file=sys.stderr, flush=True)
 
