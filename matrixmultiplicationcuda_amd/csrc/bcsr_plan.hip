@@ -364,11 +364,13 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     const bool fp8 = plan->info.precision == BCSR_PREC_FP8_E4M3;
     int tile_n = cfg ? cfg->tile_n : 0;
     if (tile_n == 0) {
-        // fastest measured on MI355X at 16384 x 16384, 5% blocks, N = 512:
-        // 128-column tiles for 16/32-row device blocks, 64 for 64-row blocks
-        // (all storage modes)
+        // fastest measured on MI355X at 16384 x 16384, 1-20% blocks:
+        // 128-column tiles for 16/32-row device blocks; for 64-row blocks
+        // 128 columns (plain kernel: the pipelined one does not fit) from
+        // N = 512 on (5%: 35.6 vs 42.4 us, 20%: 89.5 vs 134 us) and 64
+        // columns, pipelined, below (N = 256, 5%: 23.6 vs 30.0 us)
         tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
-        if (plan->rs == 4 && tile_n == 128) tile_n = 64;
+        if (plan->rs == 4 && tile_n == 128 && N < 512) tile_n = 64;
     }
     if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
     // automatic choice: the software-pipelined kernel where its register
