@@ -92,6 +92,22 @@ def precision_modes(mm, np, h16, h32, B, N, stream, ref_ms):
         ms = time_ms(stream, lambda: plan.spmm_ptr(B.ptr, C.ptr, N, stream), reps=20)
         out[prec] = {"ms": round(ms, 5), "tflops": round(flops / ms / 1e9, 2),
                      "vs_f16": round(ref_ms / ms, 3), "kernel": plan.kernel_name(N)}
+        if prec == "fp8":
+            # B quantised once (bcsr_quantize_b_e4m3), then only the product
+            # (bcsr_spmm_b8): the cost per product when B is reused
+            K = h16.block_cols * h16.bw
+            chunks = (N + 255) // 256
+            d8 = mm.DeviceArray((K * N,), np.uint8)
+            ds = mm.DeviceArray((chunks,), np.uint8)
+            work = mm.DeviceArray((K // 16 * chunks,), np.float32)
+            mm.check(mm.lib().bcsr_quantize_b_e4m3(B.ptr, K, N, d8.ptr, ds.ptr, work.ptr, stream.handle))
+            fn = lambda: mm.check(mm.lib().bcsr_spmm_b8(plan._handle, d8.ptr, ds.ptr, C.ptr, N, None,
+                                                        stream.handle))
+            ms8 = time_ms(stream, fn, reps=20)
+            out["fp8_prequantised_b"] = {"ms": round(ms8, 5), "tflops": round(flops / ms8 / 1e9, 2),
+                                         "vs_f16": round(ref_ms / ms8, 3), "kernel": plan.kernel_name(N)}
+            for a in (d8, ds, work):
+                a.free()
         C.free()
         plan.close()
     return out

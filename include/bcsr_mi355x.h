@@ -107,6 +107,14 @@ int bcsr_spmm_time(const bcsr_plan *plan, const void *B, float *C, int32_t N,
  * aligned, N % 16 == 0. */
 int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,
                          void *stream);
+/* The FP8 product on a B quantised beforehand by bcsr_quantize_b_e4m3 (K =
+ * the plan's columns, same N): only the product kernel runs, so a caller that
+ * multiplies one B by several FP8 plans, or the same B more than once,
+ * quantises it once.  FP8 plans only (BCSR_ERR_UNSUPPORTED otherwise); B8
+ * and C 16-byte aligned, N % 16 == 0; cfg may be NULL.  C is identical to
+ * bcsr_spmm's on the B the quantiser was given.  No reference counterpart. */
+int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, float *C, int32_t N,
+                 const bcsr_launch_cfg *cfg, void *stream);
 /* Name of the kernel a launch with this configuration runs (for profiling). */
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
                           char *buf, size_t buflen);

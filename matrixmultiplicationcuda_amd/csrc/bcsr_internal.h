@@ -96,6 +96,9 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
 int launch_pairs(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N,
                  hipStream_t stream);
 int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream);
+// The FP8 product on an already quantised B (bcsr_quantize_b_e4m3's layout).
+int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8_t *B8, const uint8_t *bscale,
+                       float *C, int N, hipStream_t stream);
 bool pair_shape_supported(int precision, int rs, int epl, int variant);
 void pair_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
 void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);

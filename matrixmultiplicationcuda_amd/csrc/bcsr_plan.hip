@@ -488,6 +488,20 @@ int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N, con
     return launch_spmm(plan, B, C, N, cfg, static_cast<hipStream_t>(stream));
 }
 
+int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, float *C, int32_t N,
+                 const bcsr_launch_cfg *cfg, void *stream) {
+    int rc = check_operands(plan, B8, C, N);
+    if (rc != BCSR_OK) return rc;
+    if (plan->info.precision != BCSR_PREC_FP8_E4M3) return BCSR_ERR_UNSUPPORTED;
+    if (N == 0 || plan->info.rows == 0) return BCSR_OK;
+    if (!scales) return BCSR_ERR_INVALID_ARG;
+    LaunchShape ls;
+    rc = resolve_launch(plan, N, cfg, ls);
+    if (rc != BCSR_OK) return rc;
+    return launch_fp8_product(plan, ls, static_cast<const uint8_t *>(B8), static_cast<const uint8_t *>(scales), C, N,
+                              static_cast<hipStream_t>(stream));
+}
+
 int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N, const bcsr_launch_cfg *cfg,
                      int32_t reps, void *stream) {
     if (reps < 0) return BCSR_ERR_INVALID_ARG;

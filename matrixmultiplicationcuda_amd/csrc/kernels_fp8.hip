@@ -69,9 +69,33 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
     std::snprintf(buf, len, "bcsr_spmm_fp8<%d, %d>", plan->rs, ls.epl);
 }
 
-int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream) {
+int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8_t *B8, const uint8_t *bscale,
+                       float *C, int N, hipStream_t stream) {
     Kernel k = pick_kernel(plan->rs, ls.epl);
     if (!k) return BCSR_ERR_UNSUPPORTED;
+    SpmmArgs a{};
+    a.grp_ptr = plan->d_grp_ptr;
+    a.grp_col = plan->d_grp_col;
+    a.vals = static_cast<const u32x4 *>(plan->d_vals);
+    a.ascale = plan->d_ascale;
+    a.row_order = plan->d_row_order;
+    a.B8 = B8;
+    a.b8scale = bscale;
+    a.C = C;
+    a.N = N;
+    a.n_tiles = ls.n_tiles;
+    a.dev_rows = plan->dev_rows;
+    a.map_mode = ls.map_mode;
+    a.b_bytes = static_cast<int>(static_cast<size_t>(plan->info.cols) * N);
+    a.b_chunks = (N + kChunk - 1) / kChunk;
+    a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 2048, INT32_MAX));
+    a.wpb = ls.wpb;
+    hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
+    return hip_status(hipGetLastError());
+}
+
+int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream) {
+    if (!pick_kernel(plan->rs, ls.epl)) return BCSR_ERR_UNSUPPORTED;
     const int K = plan->info.cols;
     const int slabs = K / 16;
     const int chunks = (N + kChunk - 1) / kChunk;
@@ -97,25 +121,7 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     float *part = reinterpret_cast<float *>(B8 + off_p);
     int rc = enqueue_quant(B, K, N, B8, bscale, part, stream);
     if (rc != BCSR_OK) return rc;
-    SpmmArgs a{};
-    a.grp_ptr = plan->d_grp_ptr;
-    a.grp_col = plan->d_grp_col;
-    a.vals = static_cast<const u32x4 *>(plan->d_vals);
-    a.ascale = plan->d_ascale;
-    a.row_order = plan->d_row_order;
-    a.B8 = B8;
-    a.b8scale = bscale;
-    a.C = C;
-    a.N = N;
-    a.n_tiles = ls.n_tiles;
-    a.dev_rows = plan->dev_rows;
-    a.map_mode = ls.map_mode;
-    a.b_bytes = static_cast<int>(q_bytes);
-    a.b_chunks = chunks;
-    a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 2048, INT32_MAX));
-    a.wpb = ls.wpb;
-    hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
-    return hip_status(hipGetLastError());
+    return launch_fp8_product(plan, ls, B8, bscale, C, N, stream);
 }
 
 extern "C" int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,

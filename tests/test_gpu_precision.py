@@ -109,6 +109,40 @@ def test_fp8_b_quantiser_matches_oracle():
         assert np.array_equal(got, want_q), f"call {call}"
 
 
+@pytest.mark.parametrize("bh,N", [(16, 512), (32, 96), (64, 272)])
+def test_spmm_on_prequantised_b_equals_spmm(bh, N):
+    """bcsr_spmm_b8 on bcsr_quantize_b_e4m3's output == bcsr_spmm (which
+    quantises B itself) bit for bit, over two calls with different N; a
+    16-bit plan and a misaligned B8 are refused."""
+    lib = mm.lib()
+    M, K = 512, 1024
+    A32 = mm.generate(M, K, 0.8, "blockrandom", bh=bh, seed=bh, dtype=np.float32)
+    h = mm.bcsr_from_dense(A32.astype(np.float16), bh, 16)
+    h32 = mm.HostBCSR(h.hdr, h.idx, oracle.bcsr_values_f32(A32, h.hdr, h.idx, bh, 16), h.block_rows,
+                      h.block_cols, bh, 16)
+    plan = mm.Plan(h32, "fp8")
+    for n in (N, 48):
+        B16 = mm.generate(K, n, 0.0, "random", seed=n)
+        dB = mm.DeviceArray.from_numpy(B16)
+        chunks = (n + 255) // 256
+        d8 = mm.DeviceArray((K * n,), np.uint8)
+        ds = mm.DeviceArray((chunks,), np.uint8)
+        work = mm.DeviceArray((K // 16 * chunks,), np.float32)
+        C1 = mm.DeviceArray((M, n), np.float32)
+        C2 = mm.DeviceArray((M, n), np.float32)
+        C1.fill_bytes(0xFF)
+        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, n, d8.ptr, ds.ptr, work.ptr, None))
+        mm.check(lib.bcsr_spmm_b8(plan._handle, d8.ptr, ds.ptr, C1.ptr, n, None, None))
+        plan.spmm_ptr(dB.ptr, C2.ptr, n)
+        mm.synchronize()
+        assert np.array_equal(C1.numpy(), C2.numpy()), (bh, n)
+    assert lib.bcsr_spmm_b8(plan._handle, d8.offset(8), ds.ptr, C1.ptr, 48, None, None) == -3
+    p16 = mm.Plan(h, "f16")
+    assert lib.bcsr_spmm_b8(p16._handle, d8.ptr, ds.ptr, C1.ptr, 48, None, None) == -7
+    p16.close()
+    plan.close()
+
+
 @pytest.mark.parametrize("mode", ["bf16", "hilo", "fp8"])
 def test_precision_uniform_values_target_shape_rows(mode):
     """16384^2, 5% blocks, N = 512 (U(-1,1) values): sampled block-rows."""
