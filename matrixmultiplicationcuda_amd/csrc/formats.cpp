@@ -1,7 +1,7 @@
 // formats.cpp -- host construction of the BCSR, CSR and hybrid formats
 // (formats/BCSRMatrix.cu:22-104, formats/CSRMatrix.cu:13-40,
 // formats/HCSRMatrix.cpp:10-36).  The device side of copyToDevice lives with
-// the kernels (bcsr_device.hip).
+// the plan builder (bcsr_plan.hip).
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
