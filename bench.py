@@ -379,6 +379,8 @@ def main():
         pk = MFMA_PEAK_TFLOPS[args.precision]
         roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": pk, "unit": "TFLOP/s",
                 "frac": round(tflops / pk, 4), "traffic": None}
+    # the matrix cores' share of their dense peak at this throughput
+    roof["mfma_utilisation"] = round(tflops / MFMA_PEAK_TFLOPS[args.precision], 4)
     roof["kernel"] = kname
     roof["roofline_time_us"] = round(t_star * 1e6, 3)
     roof["algorithmic_bytes"] = int(q)
