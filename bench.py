@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--N", type=int, default=512)
     p.add_argument("--density", type=float, default=0.05, help="fraction of nonzero blocks")
     p.add_argument("--bh", type=int, default=16)
+    p.add_argument("--seed", type=int, default=123, help="generator seed of A (the reference's 123)")
+    p.add_argument("--seed-b", type=int, default=456, help="generator seed of B (the reference's 456)")
     p.add_argument("--pattern", default=None,
                    help="generator pattern (default: blockrandom / pattern32by16 / pattern64by16 by bh)")
     p.add_argument("--tile-n", type=int, default=0)
@@ -328,10 +330,10 @@ def main():
     t0 = time.time()
     # the BCSR arrays of generate_matrix(M, K, 1 - density, pattern, bh, 123)
     # rounded to fp16, produced block by block (no dense M x K matrix)
-    h32 = mm.generate_bcsr(M, K, 1.0 - args.density, pattern, bh=bh, bw=bw, seed=123)
+    h32 = mm.generate_bcsr(M, K, 1.0 - args.density, pattern, bh=bh, bw=bw, seed=args.seed)
     h = mm.HostBCSR(h32.hdr, h32.idx, h32.data.astype(np.float16), h32.block_rows,
                     h32.block_cols, bh, bw)
-    B16 = mm.generate(K, N, 0.0, "random", bh=bh, seed=456)
+    B16 = mm.generate(K, N, 0.0, "random", bh=bh, seed=args.seed_b)
     mm.check(mm.lib().bcsr_set_device(0))
     # fp16 storage takes the fp16 matrix (as the reference does); the other
     # modes quantise the generator's fp32 values
@@ -410,12 +412,22 @@ def main():
             if sum(times) > 20.0:
                 break
         tc = float(np.median(times))
-        _, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data.astype(np.float32), bh, bw, Bf,
-                                       threads=threads)
+        C64, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data.astype(np.float32), bh, bw, Bf,
+                                         threads=threads)
         err = np.abs(C_gpu.astype(np.float64) - C_cpu)
         nz = Cabs > 0
         rel = float((err[nz] / Cabs[nz]).max(initial=0.0))
         exact_zero = bool(np.all(C_gpu[~nz] == 0))
+        # FP64 products: of the stored fp16 values (the kernel's arithmetic
+        # error) and of the generator's values before fp16 rounding (storage
+        # included)
+        rel64 = float((np.abs(C_gpu - C64)[nz] / Cabs[nz]).max(initial=0.0))
+        del C64
+        C64g, Cabsg = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, h32.data, bh, bw, Bf, threads=threads)
+        nzg = Cabsg > 0
+        rel64g = float((np.abs(C_gpu - C64g)[nzg] / Cabsg[nzg]).max(initial=0.0))
+        mean64g = float((np.abs(C_gpu - C64g)[nzg] / Cabsg[nzg]).mean()) if nzg.any() else 0.0
+        del C64g, Cabsg
         # fp16 storage is held to 1e-3 of |A||B| per entry; the other modes
         # store different values and are held to their own bounds in the tests
         bound = {"f16": 1e-3, "bf16": 2e-2, "hilo": 1e-3, "fp8": 0.25}[args.precision]
@@ -425,9 +437,12 @@ def main():
                          f"loop over {threads} threads, median of {len(times)} runs "
                          f"({tc:.3f} s each)",
                "seconds": round(tc, 4),
-               "gpu_vs_cpu": {"max_err_over_absAabsB": rel, "bound": 1e-3,
+               "gpu_vs_cpu": {"max_err_over_absAabsB": rel, "bound": bound,
                               "exact_zero_where_absAabsB_zero": exact_zero,
-                              "ok": rel <= 1e-3 and exact_zero}}
+                              "ok": rel <= bound and exact_zero},
+               "gpu_vs_fp64": {"max_err_over_absAabsB_stored_values": rel64,
+                               "max_err_over_absAabsB_generator_values": rel64g,
+                               "mean_err_over_absAabsB_generator_values": mean64g}}
 
     extensions = None
     if not args.no_extensions and args.precision == "f16":
@@ -451,11 +466,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.precision,
-        "data": "synthetic (reference generator, seeds 123/456)",
+        "data": f"synthetic (reference generator, seeds {args.seed}/{args.seed_b})",
         "config": {"workload": f"bcsr_spmm {M}x{K} {bh}x{bw} blocks {pattern} "
                                f"{args.density * 100:g}% blocks N={N} {args.precision}",
                    "M": M, "K": K, "N": N, "bh": bh, "bw": bw, "pattern": pattern,
-                   "block_density": args.density, "nnzb": int(h.nnzb),
+                   "block_density": args.density, "nnzb": int(h.nnzb), "seed_a": args.seed,
+                   "seed_b": args.seed_b,
                    "tile_n": args.tile_n, "xcd_map": args.xcd_map, "variant": args.variant,
                    "waves_per_block": args.waves_per_block,
                    "parallelism": "single-gpu"},

@@ -25,9 +25,14 @@ namespace {
 thread_local int g_last_hip_error = 0;
 }
 
+// A failed HIP call also leaves its error as the runtime's "last error" until
+// hipGetLastError reads it; the launches check hipGetLastError, so a failure
+// already reported here (say an allocation that did not fit) would come back
+// as the status of the next launch.  Reading it here clears it.
 int hip_status(hipError_t e) {
     if (e == hipSuccess) return BCSR_OK;
     g_last_hip_error = static_cast<int>(e);
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? BCSR_ERR_OUT_OF_MEMORY : BCSR_ERR_HIP;
 }
 
