@@ -316,8 +316,9 @@ def batched(mm, np, count, stream):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding over several GPUs (by "
+                                   "block-rows, or by columns of B and C) is not launched",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
 

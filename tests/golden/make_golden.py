@@ -9,8 +9,9 @@ __float2half) and the BCSR arrays follow formats/BCSRMatrix.cu:36-71
 (stored iff a half is nonzero, ascending block-columns, row-major blocks).
 
 Outputs
-  golden.json       config-0 hdr/idx in full, SHA-256 of every array of the
-                    config-0 and 16384^2 cases, and the 8x8 checkerboard of
+  golden.json       hdr/idx of the small case (1024^2, 10% of 16x16 blocks,
+                    seed 123) in full, SHA-256 of every array of that case
+                    and of the 16384^2 cases, and the 8x8 checkerboard of
                     matrix_generator_test.cpp.
   tiny_patterns.npz float32 values of every reference pattern on small,
                     ragged shapes (and int values for the int instantiation).

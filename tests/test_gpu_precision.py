@@ -14,7 +14,7 @@ is checked twice:
      to (|A|.|B|)_ij: fp16 2^-11, bf16 2^-8 + 2^-8 (A and the B operand are
      rounded), hi+lo 5e-5; FP8 (2^-4 per operand above E4M3's subnormal
      range) mean <= 2e-2 and max <= 0.25 on log-uniform values.
-The measured maxima are printed (config-2 reports them per mode).
+The measured maxima are printed per mode.
 """
 import sys
 

@@ -45,7 +45,7 @@ def log(msg):
 
 
 def test_config0_against_dense_cpu_matmul_ref():
-    """config-0: 1024^2 blockrandom sp 0.9 16x16 seed 123, B 1024x256 seed 456."""
+    """The small reference case: 1024^2 blockrandom sp 0.9 16x16 seed 123, B 1024x256 seed 456."""
     case = make_case(1024, 1024, 256, 0.9, "blockrandom", 16, 16)
     assert case.h.nnzb == 400
     log("config0 nnzb=400")
