@@ -3,7 +3,7 @@
 
 The values come from oracle/_ref/libmg_ref.so, the reference's own
 matrix_generator.cpp compiled by oracle/Makefile (`make -C oracle ref`), so
-the fixtures pin the reference's behaviour, not this build's.  float -> half
+the fixtures pin the reference's behaviour, not this library's.  float -> half
 rounding uses numpy's float16 conversion (round to nearest even, as
 __float2half) and the BCSR arrays follow formats/BCSRMatrix.cu:36-71
 (stored iff a half is nonzero, ascending block-columns, row-major blocks).
