@@ -14,16 +14,18 @@ Each cell times the automatic launch (hipEvents, warm-up first) and reports
 effective TFLOPS 2 nnzb bh bw N / t, the roofline time max(F / 2.5 PFLOP/s,
 Q / 8 TB/s) with Q the algorithmic bytes, its fraction, and the largest
 difference over 64 sampled rows against rocBLAS's dense product of the same
-rows (as a check of the run, not a tolerance test).
+rows (as a check of the run, not a tolerance test).  For square blocks it
+also times rocSPARSE's BSR SpMM on the same operands (a comparison row).
 """
 import argparse
+import ctypes
 import json
 import sys
 import time
 
 import numpy as np
 
-from . import DeviceArray, HostBCSR, Plan, Stream, bcsr_from_dense, check, generate, generate_bcsr, lib
+from . import DeviceArray, Event, HostBCSR, Plan, Stream, bcsr_from_dense, check, generate, generate_bcsr, lib
 
 HBM = 8.0e12
 MFMA_F16 = 2.5e15
@@ -85,6 +87,11 @@ def main(argv=None):
     for bh in [int(x) for x in args.bh.split(",")]:
         for pattern, dens, h in matrices(M, K, bh):
             plan = Plan(h, "f16")
+            plain = None
+            if h.bh == h.bw:  # rocSPARSE BSR takes square blocks
+                plain = [DeviceArray.from_numpy(np.ascontiguousarray(h.hdr, np.int32)),
+                         DeviceArray.from_numpy(np.ascontiguousarray(h.idx, np.int32)),
+                         DeviceArray.from_numpy(np.ascontiguousarray(h.data, np.float16))]
             Ad = dense_rows(h, sample)
             dA = DeviceArray.from_numpy(Ad)
             density = h.nnzb / float(h.block_rows * h.block_cols)
@@ -96,6 +103,24 @@ def main(argv=None):
                 check(lib().bcsr_dense_gemm_f16(dA.ptr, dB.ptr, Cs.ptr, len(sample), K, N, stream.handle))
                 stream.synchronize()
                 diff = float(np.abs(C.numpy()[sample] - Cs.numpy()).max(initial=0.0))
+                sp_us = None
+                if plain is not None:
+                    ref = ctypes.c_void_p()
+                    check(lib().bcsr_rocsparse_create(plain[0].ptr, plain[1].ptr, plain[2].ptr, h.block_rows,
+                                                      h.block_cols, h.nnzb, h.bh, dB.ptr, C.ptr, N,
+                                                      ctypes.byref(ref)), "rocsparse bsr")
+                    run = lambda: check(lib().bcsr_rocsparse_run(ref, stream.handle))
+                    for _ in range(3):
+                        run()
+                    stream.synchronize()
+                    e0, e1 = Event(), Event()
+                    e0.record(stream)
+                    for _ in range(args.reps):
+                        run()
+                    e1.record(stream)
+                    stream.synchronize()
+                    sp_us = e0.elapsed_ms(e1) / args.reps * 1e3
+                    lib().bcsr_rocsparse_destroy(ref)
                 F = 2.0 * h.nnzb * h.bh * h.bw * N
                 Q = algorithmic_bytes(h.nnzb, h.bh, h.bw, M, K, N)
                 t_star = max(F / MFMA_F16, Q / HBM)
@@ -105,12 +130,16 @@ def main(argv=None):
                        "effective_tflops": round(F / t / 1e12, 2), "gbs": round(Q / t / 1e9, 1),
                        "bound": "mfma" if F / MFMA_F16 > Q / HBM else "hbm",
                        "roofline_us": round(t_star * 1e6, 3), "roofline_frac": round(t_star / t, 4),
-                       "max_abs_diff_sampled_rows_vs_rocblas": diff, "kernel": plan.kernel_name(N)}
+                       "max_abs_diff_sampled_rows_vs_rocblas": diff, "kernel": plan.kernel_name(N),
+                       "rocsparse_bsr_us": None if sp_us is None else round(sp_us, 3),
+                       "speedup_vs_rocsparse_bsr": None if sp_us is None else round(sp_us / (ms * 1e3), 2)}
                 rows.append(row)
                 print(json.dumps(row), flush=True)
                 C.free()
                 Cs.free()
             dA.free()
+            for a in plain or []:
+                a.free()
             plan.close()
     for _, dB in Bs.values():
         dB.free()
@@ -119,11 +148,13 @@ def main(argv=None):
         json.dump({"M": M, "K": K, "rows": rows, "seconds": round(time.time() - t_start, 1)}, f, indent=1)
     if args.md:
         with open(args.md, "w") as f:
-            f.write("| block | pattern | block density | N | time us | eff. TFLOPS | GB/s | bound | roofline frac | kernel |\n")
-            f.write("|---|---|---|---|---|---|---|---|---|---|\n")
+            f.write("| block | pattern | block density | N | time us | eff. TFLOPS | GB/s | bound | roofline frac | "
+                    "rocSPARSE BSR us (speedup) | kernel |\n")
+            f.write("|---|---|---|---|---|---|---|---|---|---|---|\n")
             for r in rows:
+                sp = "-" if r["rocsparse_bsr_us"] is None else f"{r['rocsparse_bsr_us']} ({r['speedup_vs_rocsparse_bsr']}x)"
                 f.write(f"| {r['bh']}x16 | {r['pattern']} | {r['block_density'] * 100:.2f}% | {r['N']} | {r['us']} | "
-                        f"{r['effective_tflops']} | {r['gbs']} | {r['bound']} | {r['roofline_frac']} | "
+                        f"{r['effective_tflops']} | {r['gbs']} | {r['bound']} | {r['roofline_frac']} | {sp} | "
                         f"{r['kernel']} |\n")
     return 0
 
