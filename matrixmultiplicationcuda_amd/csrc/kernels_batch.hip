@@ -198,7 +198,7 @@ extern "C" int bcsr_batch_tile_supported(int32_t precision, int32_t dev_bh, int3
 extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C,
                                  const int32_t *N, const int32_t *tile_n, int32_t count,
                                  const bcsr_batch_cfg *cfg, bcsr_batch **out) {
-    if (!out || count < 0 || (count > 0 && (!plans || !B || !C || !N))) return BCSR_ERR_INVALID_ARG;
+    if (!out || count < 0 || (count > 0 && (!plans || !B || !C || !N || !plans[0]))) return BCSR_ERR_INVALID_ARG;
     *out = nullptr;
     int dev = 0;
     int rc = hip_status(hipGetDevice(&dev));

@@ -158,6 +158,13 @@ def test_plan_validation_before_device_work():
     assert lib.bcsr_release_device(ctypes.c_void_p(0x1000)) == -6
     assert lib.sparseMatrixMulTensor(ctypes.c_void_p(0x1000), None, None, None, None, 16, 16,
                                      None) == -6
+    # a batch whose plan list holds a null plan
+    plans = (ctypes.c_void_p * 1)(None)
+    ptrs = (ctypes.c_void_p * 1)(None)
+    n = np.array([16], dtype=np.int32)
+    batch = ctypes.c_void_p()
+    assert lib.bcsr_batch_create(plans, ptrs, ptrs, n.ctypes.data_as(ctypes.c_void_p), None, 1, None,
+                                 ctypes.byref(batch)) == -1
 
 
 def test_mat_roundtrip(tmp_path):
