@@ -172,7 +172,8 @@ int bcsr_cm_num_params(const bcsr_costmodel *cm, int32_t *n);
 int bcsr_cm_set_params(bcsr_costmodel *cm, const float *host, int32_t n); /* resets Adam */
 int bcsr_cm_get_params(const bcsr_costmodel *cm, float *host, int32_t n);
 int bcsr_cm_get_grads(const bcsr_costmodel *cm, float *host, int32_t n);
-/* P [n][out] = model(X). */
+/* P [n][out] = model(X).  n (rows) is at most 2^24 per call in forward,
+ * backward and train (BCSR_ERR_INVALID_ARG above). */
 int bcsr_cm_forward(bcsr_costmodel *cm, const float *X, int32_t n, float *P, void *stream);
 /* Gradient of the loss at the current parameters (read with get_grads);
  * *loss (optional) = the loss, which synchronises the stream. */

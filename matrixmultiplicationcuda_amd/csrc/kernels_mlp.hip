@@ -136,6 +136,9 @@ __global__ __launch_bounds__(256) void mlp_adam(float *__restrict__ p, const flo
 
 unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
+// Rows per call: 2^24 rows x 4096 units keeps every grid within 2^28 blocks.
+constexpr int32_t kMaxRows = 1 << 24;
+
 }  // namespace
 
 struct bcsr_costmodel {
@@ -322,7 +325,7 @@ extern "C" int bcsr_cm_get_grads(const bcsr_costmodel *cm, float *host, int32_t 
 }
 
 extern "C" int bcsr_cm_forward(bcsr_costmodel *cm, const float *X, int32_t n, float *P, void *stream) {
-    if (!cm || n < 0 || (n && (!X || !P))) return BCSR_ERR_INVALID_ARG;
+    if (!cm || n < 0 || n > kMaxRows || (n && (!X || !P))) return BCSR_ERR_INVALID_ARG;
     if (!n) return BCSR_OK;
     int rc = ensure_ws(cm, n);
     if (rc != BCSR_OK) return rc;
@@ -333,7 +336,7 @@ extern "C" int bcsr_cm_forward(bcsr_costmodel *cm, const float *X, int32_t n, fl
 // gradient buffer) and the loss (*loss, synchronises the stream).
 extern "C" int bcsr_cm_backward(bcsr_costmodel *cm, const float *X, const float *Y, const float *M, int32_t n,
                                 float inv_count, float *loss, void *stream) {
-    if (!cm || n <= 0 || !X || !Y || !M || !(inv_count > 0.0f)) return BCSR_ERR_INVALID_ARG;
+    if (!cm || n <= 0 || n > kMaxRows || !X || !Y || !M || !(inv_count > 0.0f)) return BCSR_ERR_INVALID_ARG;
     int rc = ensure_ws(cm, n);
     if (rc != BCSR_OK) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -349,7 +352,7 @@ extern "C" int bcsr_cm_backward(bcsr_costmodel *cm, const float *X, const float 
 extern "C" int bcsr_cm_train(bcsr_costmodel *cm, const float *X, const float *Y, const float *M, int32_t n,
                              float inv_count, int32_t steps, float lr, float beta1, float beta2, float eps,
                              float *loss, void *stream) {
-    if (!cm || n <= 0 || steps < 0 || !X || !Y || !M || !(inv_count > 0.0f) || !(lr > 0.0f))
+    if (!cm || n <= 0 || n > kMaxRows || steps < 0 || !X || !Y || !M || !(inv_count > 0.0f) || !(lr > 0.0f))
         return BCSR_ERR_INVALID_ARG;
     int rc = ensure_ws(cm, n);
     if (rc != BCSR_OK) return rc;
