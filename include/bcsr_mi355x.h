@@ -51,7 +51,10 @@ typedef struct bcsr_plan bcsr_plan;
 typedef struct {
     int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 */
     int32_t xcd_map;  /* -1 = auto, 0 = linear block order, 1 = column tiles pinned to XCDs */
-    int32_t variant;  /* -1 = auto; 0 plain, 1 double-buffered, 2 64-bit addressing (16-bit modes) */
+    int32_t variant;  /* -1 = auto; 0 plain, 1 double-buffered, 2 64-bit addressing (16-bit modes);
+                       * 3 = panel kernel: B slabs staged in LDS and shared by 16 block-rows
+                       * (fp16 / bf16, 16-row device blocks, 128-column tiles only; never
+                       * chosen automatically) */
     int32_t waves_per_block; /* 0 = auto; 1, 2 or 4 waves (independent work items) per workgroup */
     int32_t reserved[4];
 } bcsr_launch_cfg;

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "bcsr_mi355x.h"
+#include "panel_schedule.h"
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -59,6 +60,24 @@ struct bcsr_plan {
     size_t ws_bytes = 0;
     // host copies of grp_ptr and row_order (work-item construction)
     std::vector<int32_t> h_grp_ptr, h_row_order;
+    // panel schedule (16-row device blocks, fp16 / bf16 only; n_panels = 0
+    // otherwise): chunk ranges per panel of 16 device block-rows and one
+    // record of kPanelRecInts ints per chunk (layout in kernels_panel.hip)
+    int n_panels = 0;
+    int32_t *d_panel_ptr = nullptr;
+    int32_t *d_panel_rec = nullptr;
+    int64_t panel_chunks = 0;
+};
+
+
+struct PanelArgs {
+    const int32_t *chunk_ptr;  // panels + 1
+    const int32_t *records;    // kPanelRecInts per chunk (+ one spare)
+    const void *vals;          // the plan's pair operands
+    const _Float16 *B;
+    float *C;
+    int N, n_tiles, panels, dev_rows, map_mode;
+    int b_bytes, a_bytes, rec_bytes;
 };
 
 // Launch geometry resolved from a bcsr_launch_cfg.
@@ -100,6 +119,11 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
 int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8_t *B8, const uint8_t *bscale,
                        float *C, int N, hipStream_t stream);
 bool pair_shape_supported(int precision, int rs, int epl, int variant);
+// The slab-streaming panel kernel (variant 3).
+bool panel_supported(const bcsr_plan *plan);
+int panel_grid(const bcsr_plan *plan, int n_tiles, int map_mode);
+int launch_panel(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream);
+void panel_kernel_name(const bcsr_plan *plan, char *buf, size_t len);
 void pair_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
 void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
 

@@ -237,6 +237,15 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
         }
     }
 
+    // panel schedule for 16-row device blocks in the 16-bit single-operand modes
+    std::vector<int32_t> panel_ptr, panel_rec;
+    if (!fp8 && H == 1 && rs == 1 && groups * 1024 <= INT32_MAX && dev_rows > 0) {
+        std::vector<int32_t> nblk(dev_rows);
+        for (int r = 0; r < dev_rows; ++r)
+            nblk[r] = (hdr[r / sub_rows + 1] - hdr[r / sub_rows]) * pieces;
+        build_panel_schedule(grp_ptr, grp_col, nblk, dev_rows, panel_ptr, panel_rec);
+    }
+
     std::vector<int32_t> row_order(dev_rows);
     for (int r = 0; r < dev_rows; ++r) row_order[r] = r;
     std::stable_sort(row_order.begin(), row_order.end(),
@@ -252,7 +261,9 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     const size_t off_ro = align_up(off_gc + sizeof(int32_t) * std::max<size_t>(grp_col.size(), 4), A);
     const size_t off_as = align_up(off_ro + sizeof(int32_t) * std::max(dev_rows, 1), A);
     const size_t off_v = align_up(off_as + sizeof(uint32_t) * std::max<size_t>(ascale.size(), 1), A);
-    const size_t total = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
+    const size_t off_pp = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
+    const size_t off_pr = align_up(off_pp + sizeof(int32_t) * panel_ptr.size(), A);
+    const size_t total = align_up(off_pr + sizeof(int32_t) * panel_rec.size(), A);
 
     int dev = 0;
     rc = hip_status(hipGetDevice(&dev));
@@ -280,7 +291,9 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
                   {plan->d_grp_col, grp_col.data(), sizeof(int32_t) * grp_col.size()},
                   {plan->d_row_order, row_order.data(), sizeof(int32_t) * dev_rows},
                   {base + off_as, ascale.data(), sizeof(uint32_t) * ascale.size()},
-                  {plan->d_vals, vals.data(), vals.size()}};
+                  {plan->d_vals, vals.data(), vals.size()},
+                  {base + off_pp, panel_ptr.data(), sizeof(int32_t) * panel_ptr.size()},
+                  {base + off_pr, panel_rec.data(), sizeof(int32_t) * panel_rec.size()}};
     for (const Copy &c : copies) {
         if (!c.bytes) continue;
         rc = hip_status(hipMemcpy(c.dst, c.src, c.bytes, hipMemcpyHostToDevice));
@@ -289,6 +302,12 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
             delete plan;
             return rc;
         }
+    }
+    if (!panel_ptr.empty()) {
+        plan->n_panels = static_cast<int>(panel_ptr.size()) - 1;
+        plan->d_panel_ptr = reinterpret_cast<int32_t *>(base + off_pp);
+        plan->d_panel_rec = reinterpret_cast<int32_t *>(base + off_pr);
+        plan->panel_chunks = panel_ptr.back();
     }
     plan->rs = rs;
     plan->dev_rows = dev_rows;
@@ -332,6 +351,7 @@ int launch_spmm(const bcsr_plan *plan, const void *B, float *C, int N, const bcs
     if (rc != BCSR_OK) return rc;
     if (plan->info.precision == BCSR_PREC_FP8_E4M3)
         return launch_fp8(const_cast<bcsr_plan *>(plan), ls, B, C, N, stream);
+    if (ls.variant == 3) return launch_panel(plan, ls, B, C, N, stream);
     return launch_pairs(plan, ls, B, C, N, stream);
 }
 
@@ -382,6 +402,21 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     // footprint fits, else the plain one
     const bool auto_variant = !cfg || cfg->variant < 0;
     ls.variant = auto_variant ? 1 : cfg->variant;
+    if (ls.variant == 3) {
+        // the slab-streaming panel kernel: 128-column tiles, 1024 threads
+        if (!panel_supported(plan) || tile_n != 128) return BCSR_ERR_UNSUPPORTED;
+        if (static_cast<int64_t>(plan->info.cols) * N * 2 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;
+        ls.epl = 8;
+        ls.n_tiles = (N + 127) / 128;
+        const bool pin_ok = ls.n_tiles == 1 || ls.n_tiles == 2 || ls.n_tiles == 4 || ls.n_tiles % 8 == 0;
+        int pmap = cfg ? cfg->xcd_map : -1;
+        if (pmap < 0) pmap = pin_ok ? 1 : 0;
+        if ((pmap == 1 && !pin_ok) || (pmap != 0 && pmap != 1)) return BCSR_ERR_INVALID_ARG;
+        ls.map_mode = pmap;
+        ls.wpb = 16;
+        ls.grid = panel_grid(plan, ls.n_tiles, pmap);
+        return BCSR_OK;
+    }
     if (ls.variant > 2) return BCSR_ERR_UNSUPPORTED;
     if (!fp8 && !pair_shape_supported(plan->info.precision, plan->rs, tile_n / 16, ls.variant)) {
         if (!auto_variant || ls.variant != 1) return BCSR_ERR_UNSUPPORTED;
@@ -546,6 +581,8 @@ int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cf
     if (rc != BCSR_OK) return rc;
     if (plan->info.precision == BCSR_PREC_FP8_E4M3)
         fp8_kernel_name(plan, ls, buf, buflen);
+    else if (ls.variant == 3)
+        panel_kernel_name(plan, buf, buflen);
     else
         pair_kernel_name(plan, ls, buf, buflen);
     return BCSR_OK;
