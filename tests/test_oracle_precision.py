@@ -3,7 +3,7 @@ scales, hi+lo fp16).  These modes have no reference counterpart; each oracle
 function is checked against an independent implementation (PyTorch's CPU
 float8_e4m3fn / bfloat16 casts, numpy float16) or against the closed form the
 format fixes, so that a wrong bias, rounding mode, scale direction or slab
-index fails here before it is used to judge the kernels.
+index fails here before it is used to check the kernels.
 """
 import math
 
