@@ -188,7 +188,11 @@ def choose(pred_log, mask, spread=FALLBACK_SPREAD):
     p = np.where(mask > 0, pred_log, np.inf)
     k = int(np.argmin(p))
     sup = pred_log[mask > 0]
-    fallback = bool(sup.size > 1 and math.exp(float(sup.max() - sup.min())) - 1.0 < spread)
+    if sup.size and not np.all(np.isfinite(sup)):
+        return k, True  # a prediction that is not a number: autotune instead
+    # spread of predicted times exp(max - min) - 1, compared in log space
+    # (a badly trained model can predict log-time gaps far beyond exp's range)
+    fallback = bool(sup.size > 1 and float(sup.max() - sup.min()) < math.log1p(spread))
     return k, fallback
 
 

@@ -285,3 +285,26 @@ def test_cpp_api_compiles_and_runs(tmp_path):
     blk0 = A16[:16, :16].astype(np.float32)
     assert float(out[5]) == pytest.approx(np.count_nonzero(blk0) / 256, abs=1e-4)
     assert out[6] == "1"
+
+
+def test_dispatcher_choice_edge_cases():
+    """learned.choose: the predicted-fastest supported configuration, and the
+    autotune fallback when the predicted spread is under 5 % -- also for
+    predictions far outside exp's range and for predictions that are NaN."""
+    import math
+
+    from matrixmultiplicationcuda_amd import learned
+
+    mask = np.array([1, 1, 0, 1], np.float32)
+    k, fb = learned.choose(np.array([2.0, 1.0, -5.0, 3.0]), mask)
+    assert k == 1 and not fb  # masked-out config 2 is never chosen
+    k, fb = learned.choose(np.array([1.0, 1.0 + math.log(1.04), 0.0, 1.01]), mask)
+    assert k == 0 and fb  # all within 4 %: autotune
+    k, fb = learned.choose(np.array([1.0, 1.0 + math.log(1.06), 0.0, 1.0]), mask)
+    assert not fb  # 6 % apart: trust the model
+    k, fb = learned.choose(np.array([-800.0, 900.0, 0.0, 5.0]), mask)
+    assert k == 0 and not fb  # no overflow for a 1700-nat spread
+    k, fb = learned.choose(np.array([1.0, np.nan, 0.0, 2.0]), mask)
+    assert fb
+    k, fb = learned.choose(np.array([1.0, 2.0, 0.0, 3.0]), np.array([0, 1, 0, 0], np.float32))
+    assert k == 1 and not fb  # a single supported configuration
