@@ -15,6 +15,10 @@
 // two ahead, so the loop reads its metadata from LDS: the loads of chunk
 // c + 2 are addressed from the record that arrived with chunk c, and each
 // wave keeps its own row's fields of the next two chunks in registers.
+// The kernel runs only when a launch configuration asks for variant 3: at
+// the shapes measured on MI355X it is slower than the pair kernels, because
+// its per-chunk load latency, barrier and LDS read burst add up instead of
+// overlapping.
 //
 // A chunk record (kPanelRecInts = 96 ints, built on the host with the plan):
 //   [0, 8)        block-columns of the chunk's slabs (-1: none)
