@@ -132,7 +132,9 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
 // dispatched round-robin over the 8 XCDs).  map_mode 0 ignores the XCD and
 // numbers items b * wpb + w, tile fastest; map_mode 1 pins every column tile
 // to XCDs: n_tiles % 8 == 0 gives XCD x the tiles x, x + 8, ...; n_tiles in
-// {1, 2, 4} lets 8 / n_tiles XCDs share each tile.
+// {1, 2, 4} lets 8 / n_tiles XCDs share each tile.  map_mode 2 pins rows
+// instead: XCD x takes the slots x, x + 8, ... with all their column tiles,
+// so a block-row's A is fetched into one XCD's L2 for every tile.
 __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
     // threadIdx.x >> 6 is the same in every lane of a wave; readfirstlane
     // tells the compiler so, which keeps everything derived from it (row,
@@ -142,6 +144,10 @@ __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot
         const int item = bid * a.wpb + w;
         tile = item % a.n_tiles;
         slot = item / a.n_tiles;
+    } else if (a.map_mode == 2) {
+        const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
+        tile = i % a.n_tiles;
+        slot = (i / a.n_tiles) * 8 + x;
     } else {
         const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
         if (a.n_tiles >= 8) {

@@ -433,9 +433,22 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     ls.n_tiles = (N + tile_n - 1) / tile_n;
     const bool xcd_ok = ls.n_tiles == 1 || ls.n_tiles == 2 || ls.n_tiles == 4 || ls.n_tiles % 8 == 0;
     int map = cfg ? cfg->xcd_map : -1;
-    if (map < 0) map = xcd_ok ? 1 : 0;
+    if (map < 0) {
+        // Column tiles pinned to XCDs keep each XCD's slice of B in its L2;
+        // rows pinned fetch each block-row's A into one L2 for all tiles.
+        // Measured on MI355X at 16384^2: rows win for 64-row device blocks
+        // (N = 256: 1.15-1.34x, N = 512 at 20 %: 1.11x) and for dense
+        // matrices with at most two tiles (20 %, N = 256: 1.04-1.11x); tiles
+        // win elsewhere (16 x 16, 5 %, N = 512: 52 vs 84 us).
+        const double density = static_cast<double>(plan->info.nnzb) /
+                               std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
+        if (plan->rs == 4 || (ls.n_tiles <= 2 && density >= 0.15))
+            map = 2;
+        else
+            map = xcd_ok ? 1 : 0;
+    }
     if (map == 1 && !xcd_ok) return BCSR_ERR_INVALID_ARG;
-    if (map != 0 && map != 1) return BCSR_ERR_INVALID_ARG;
+    if (map < 0 || map > 2) return BCSR_ERR_INVALID_ARG;
     ls.map_mode = map;
     const int wpb = cfg && cfg->waves_per_block > 0 ? cfg->waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return BCSR_ERR_INVALID_ARG;
@@ -443,6 +456,9 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     const int rows = plan->dev_rows;
     if (map == 0) {
         ls.grid = (ls.n_tiles * rows + wpb - 1) / wpb;
+    } else if (map == 2) {
+        const int per_xcd = (rows + 7) / 8 * ls.n_tiles;  // items of one XCD
+        ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
     } else if (ls.n_tiles >= 8) {
         const int per_xcd = ls.n_tiles / 8 * rows;  // items of one XCD
         ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);

@@ -166,3 +166,33 @@ def test_precision_uniform_values_target_shape_rows(mode):
         worst = max(worst, rel)
     log(f"  {mode} target rows: max rel |A||B| {worst:.3e}")
     assert worst <= {"bf16": 2e-2, "hilo": 5e-5, "fp8": 2e-2}[mode]
+
+
+@pytest.mark.parametrize("xcd_map", [0, 1, 2])
+@pytest.mark.parametrize("bh,N", [(16, 512), (64, 256), (32, 384)])
+def test_fp8_xcd_mappings_agree(bh, N, xcd_map):
+    """The FP8 kernel under every work-to-XCD mapping gives the automatic
+    launch's C bit for bit (the mapping only moves work items)."""
+    M, K = 1024, 1024
+    A32 = mm.generate(M, K, 0.8, "blockrandom", bh=bh, seed=bh + 1, dtype=np.float32)
+    h = mm.bcsr_from_dense(A32.astype(np.float16), bh, 16)
+    h32 = mm.HostBCSR(h.hdr, h.idx, oracle.bcsr_values_f32(A32, h.hdr, h.idx, bh, 16), h.block_rows,
+                      h.block_cols, bh, 16)
+    plan = mm.Plan(h32, "fp8")
+    B16 = mm.generate(K, N, 0.0, "random", seed=7)
+    dB = mm.DeviceArray.from_numpy(B16)
+    C1 = mm.DeviceArray((M, N), np.float32)
+    C2 = mm.DeviceArray((M, N), np.float32)
+    C1.fill_bytes(0xFF)
+    C2.fill_bytes(0xFF)
+    plan.spmm_ptr(dB.ptr, C1.ptr, N)
+    cfg = mm.make_cfg(xcd_map=xcd_map)
+    if xcd_map == 1 and (N + 127) // 128 == 3:  # three tiles cannot be pinned
+        with pytest.raises(mm.BcsrError) as e:
+            plan.spmm_ptr(dB.ptr, C2.ptr, N, None, cfg)
+        assert e.value.status == -1
+    else:
+        plan.spmm_ptr(dB.ptr, C2.ptr, N, None, cfg)
+        mm.synchronize()
+        assert np.array_equal(C1.numpy(), C2.numpy())
+    plan.close()
