@@ -48,10 +48,7 @@ __global__ __launch_bounds__(256) void csr_spmm(const int *__restrict__ hdr, con
         acc[3] = c[3];
     }
     const _Float16 *bcol = B + (valid ? n0 : 0);
-    for (int p = p0; p < p1; ++p) {
-        const int k = idx[p];
-        const float a = static_cast<float>(data[p]);
-        const u32x2 b = *reinterpret_cast<const u32x2 *>(bcol + static_cast<size_t>(k) * N);
+    auto step = [&](float a, u32x2 b) {
         const float b0 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(b.x & 0xffffu)));
         const float b1 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(b.x >> 16)));
         const float b2 = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(b.y & 0xffffu)));
@@ -60,7 +57,27 @@ __global__ __launch_bounds__(256) void csr_spmm(const int *__restrict__ hdr, con
         acc[1] = fmaf(a, b1, acc[1]);
         acc[2] = fmaf(a, b2, acc[2]);
         acc[3] = fmaf(a, b3, acc[3]);
+    };
+    // eight nonzeros per step: their index, value and B loads are all in
+    // flight together, and the fmas still run in CSR order (bit-exact)
+    constexpr int U = 8;
+    int p = p0;
+    for (; p + U <= p1; p += U) {
+        int k[U];
+        float a[U];
+        u32x2 b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            k[u] = idx[p + u];
+            a[u] = static_cast<float>(data[p + u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) b[u] = *reinterpret_cast<const u32x2 *>(bcol + static_cast<size_t>(k[u]) * N);
+#pragma unroll
+        for (int u = 0; u < U; ++u) step(a[u], b[u]);
     }
+    for (; p < p1; ++p)
+        step(static_cast<float>(data[p]), *reinterpret_cast<const u32x2 *>(bcol + static_cast<size_t>(idx[p]) * N));
     if (!valid) return;
     f32x4 out{acc[0], acc[1], acc[2], acc[3]};
     if (accumulate == 1) out += *cp;
