@@ -396,10 +396,20 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // columns, pipelined, below (N = 256, 5%: 23.6 vs 30.0 us)
         tile_n = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
         if (plan->rs == 4 && tile_n == 128 && N < 512) tile_n = 64;
-        // 64-row blocks at N <= 128 (16-bit modes): 32-column tiles give
-        // twice the waves; N = 64: 4.6 vs 5.2 us (1 %), 11.9 vs 13.1 (5 %),
-        // 32.5 vs 37.1 (20 %); N = 128: 1-8 % faster
-        if (plan->rs == 4 && N <= 128 && !fp8) tile_n = 32;
+        // 64-row blocks at N <= 128: 32-column tiles give twice the waves;
+        // fp16 N = 64: 4.6 vs 5.2 us (1 %), 11.9 vs 13.1 (5 %), 32.5 vs 37.1
+        // (20 %); N = 128 and FP8: 2-14 % faster
+        if (plan->rs == 4 && N <= 128) tile_n = 32;
+        if (fp8) {
+            // FP8, 32-row blocks at N = 128: 64-column tiles (18.1 vs 18.6 us
+            // at 5 %, 28.3 vs 31.4 at 20 %); 16/32-row blocks at N >= 512
+            // with >= 15 % blocks: 256-column tiles (20 %, N = 512: 84.6 vs
+            // 90.6 us for 16-row blocks, 59.1 vs 63.5 for 32-row blocks)
+            const double density = static_cast<double>(plan->info.nnzb) /
+                                   std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
+            if (plan->rs == 2 && N > 64 && N <= 128) tile_n = 64;
+            if (plan->rs <= 2 && N >= 512 && density >= 0.15) tile_n = 256;
+        }
     }
     if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
     // automatic choice: the software-pipelined kernel where its register
