@@ -400,13 +400,19 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // fp16 N = 64: 4.6 vs 5.2 us (1 %), 11.9 vs 13.1 (5 %), 32.5 vs 37.1
         // (20 %); N = 128 and FP8: 2-14 % faster
         if (plan->rs == 4 && N <= 128) tile_n = 32;
+        const double density = static_cast<double>(plan->info.nnzb) /
+                               std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
+        // 16-row blocks in the 16-bit modes at N >= 512 with >= 8 % blocks:
+        // 256-column tiles (bench.py at 16384^2, N = 512: 10 %: 81.3 vs
+        // 85.1 us, 20 %: 147.6 vs 152.9 us; at 5 % 128 columns stay ahead,
+        // 49.5 vs 51.4 us)
+        if (!fp8 && plan->rs == 1 && plan->info.precision != BCSR_PREC_F16_HILO && N >= 512 && density >= 0.08)
+            tile_n = 256;
         if (fp8) {
             // FP8, 32-row blocks at N = 128: 64-column tiles (18.1 vs 18.6 us
             // at 5 %, 28.3 vs 31.4 at 20 %); 16/32-row blocks at N >= 512
             // with >= 15 % blocks: 256-column tiles (20 %, N = 512: 84.6 vs
             // 90.6 us for 16-row blocks, 59.1 vs 63.5 for 32-row blocks)
-            const double density = static_cast<double>(plan->info.nnzb) /
-                                   std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
             if (plan->rs == 2 && N > 64 && N <= 128) tile_n = 64;
             if (plan->rs <= 2 && N >= 512 && density >= 0.15) tile_n = 256;
         }
@@ -452,11 +458,12 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // rows pinned fetch each block-row's A into one L2 for all tiles.
         // Measured on MI355X at 16384^2: rows win for 64-row device blocks
         // (N = 256: 1.15-1.34x, N = 512 at 20 %: 1.11x) and for dense
-        // matrices with at most two tiles (20 %, N = 256: 1.04-1.11x); tiles
-        // win elsewhere (16 x 16, 5 %, N = 512: 52 vs 84 us).
+        // matrices with at most two tiles of <= 128 columns (20 %, N = 256:
+        // 1.04-1.11x); tiles win elsewhere (16 x 16, 5 %, N = 512: 52 vs 84
+        // us; two 256-column tiles at 20 %: 143 vs 165 us).
         const double density = static_cast<double>(plan->info.nnzb) /
                                std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
-        if (plan->rs == 4 || (ls.n_tiles <= 2 && density >= 0.15))
+        if (plan->rs == 4 || (ls.n_tiles <= 2 && tile_n <= 128 && density >= 0.15))
             map = 2;
         else
             map = xcd_ok ? 1 : 0;
