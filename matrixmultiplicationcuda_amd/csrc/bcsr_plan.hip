@@ -388,6 +388,7 @@ int entry(int bh, const int *hdr, const int *idx, const void *data, const void *
 int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls) {
     const bool fp8 = plan->info.precision == BCSR_PREC_FP8_E4M3;
     int tile_n = cfg ? cfg->tile_n : 0;
+    if (tile_n == 0 && cfg && cfg->variant == 3) tile_n = 128;  // the panel kernel's only width
     if (tile_n == 0) {
         // fastest measured on MI355X at 16384 x 16384, 1-20% blocks:
         // 128-column tiles for 16/32-row device blocks; for 64-row blocks

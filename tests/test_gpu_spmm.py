@@ -243,3 +243,28 @@ def test_target_scale_sampled_rows_and_checksum():
     got = Cg.astype(np.float64).sum(axis=1)
     assert np.all(np.abs(got - rows_ref) <= 1e-3 * rows_abs + 1e-6)
     plan.close()
+
+
+@pytest.mark.parametrize("bh,density,N,prec,want", [
+    (16, 0.05, 512, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),     # the target: 128-column tiles
+    (16, 0.20, 512, "f16", "bcsr_spmm_pair<1, 16, true, f16>"),    # dense: 256-column tiles
+    (16, 0.20, 512, "bf16", "bcsr_spmm_pair<1, 16, true, bf16>"),
+    (16, 0.20, 512, "hilo", "bcsr_spmm_pair<1, 8, true, hilo>"),   # hi+lo keeps 128
+    (64, 0.05, 64, "f16", "bcsr_spmm_pair<4, 2, true, f16>"),      # 64-row blocks, N <= 128: 32 columns
+    (64, 0.05, 256, "f16", "bcsr_spmm_pair<4, 4, true, f16>"),
+    (64, 0.05, 512, "f16", "bcsr_spmm_pair<4, 8, false, f16>"),
+    (32, 0.20, 128, "fp8", "bcsr_spmm_fp8<2, 4>"),                 # FP8, 32-row blocks, N = 128: 64 columns
+    (16, 0.20, 512, "fp8", "bcsr_spmm_fp8<1, 16>"),                # FP8 dense: 256 columns
+    (16, 0.05, 512, "fp8", "bcsr_spmm_fp8<1, 8>"),
+])
+def test_automatic_choices(bh, density, N, prec, want):
+    """The automatic launch picks the tile widths measured fastest (kernel
+    names carry the block sub-tiles and 16-column groups per wave)."""
+    pattern = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}[bh]
+    M = K = 2048
+    h = mm.generate_bcsr(M, K, 1.0 - density, pattern, bh=bh, bw=16, seed=5)
+    if prec == "f16":
+        h = mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, bh, 16)
+    plan = mm.Plan(h, prec)
+    assert plan.kernel_name(N) == want
+    plan.close()
