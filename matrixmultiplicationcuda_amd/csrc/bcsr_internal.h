@@ -52,6 +52,7 @@ struct bcsr_plan {
     int32_t *d_grp_ptr = nullptr;
     int32_t *d_grp_col = nullptr;
     int32_t *d_row_order = nullptr;
+    int32_t *d_mix_order = nullptr;  // row_order taken longest, shortest, 2nd longest, ...
     void *d_vals = nullptr;
     uint32_t *d_ascale = nullptr;
     // FP8: workspace holding the quantised B of the current call
@@ -134,7 +135,11 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
 // to XCDs: n_tiles % 8 == 0 gives XCD x the tiles x, x + 8, ...; n_tiles in
 // {1, 2, 4} lets 8 / n_tiles XCDs share each tile.  map_mode 2 pins rows
 // instead: XCD x takes the slots x, x + 8, ... with all their column tiles,
-// so a block-row's A is fetched into one XCD's L2 for every tile.
+// so a block-row's A is fetched into one XCD's L2 for every tile.  map_mode 3
+// is for tile counts that 8 does not divide: the items in tile-major order
+// (rows in the plan's mixed order, so that any run of them carries about the
+// average work) are cut into 8 equal runs, one per XCD, so each XCD serves
+// at most two column tiles.
 __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
     // threadIdx.x >> 6 is the same in every lane of a wave; readfirstlane
     // tells the compiler so, which keeps everything derived from it (row,
@@ -148,6 +153,14 @@ __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot
         const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
         tile = i % a.n_tiles;
         slot = (i / a.n_tiles) * 8 + x;
+    } else if (a.map_mode == 3) {
+        const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
+        const int total = a.n_tiles * a.dev_rows, per = (total + 7) / 8;  // total < 2^31 (checked at launch)
+        if (i >= per) return false;
+        const int64_t g = static_cast<int64_t>(x) * per + i;
+        if (g >= total) return false;
+        tile = static_cast<int>(g / a.dev_rows);
+        slot = static_cast<int>(g % a.dev_rows);
     } else {
         const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
         if (a.n_tiles >= 8) {

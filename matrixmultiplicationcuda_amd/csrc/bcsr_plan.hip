@@ -250,6 +250,10 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     for (int r = 0; r < dev_rows; ++r) row_order[r] = r;
     std::stable_sort(row_order.begin(), row_order.end(),
                      [&](int x, int y) { return grp_ptr[x + 1] - grp_ptr[x] > grp_ptr[y + 1] - grp_ptr[y]; });
+    // the same rows taken from both ends in turn: any run of consecutive
+    // entries then holds about the average work (chunked mapping)
+    std::vector<int32_t> mix_order(dev_rows);
+    for (int k = 0, lo = 0, hi = dev_rows - 1; k < dev_rows; ++k) mix_order[k] = row_order[k % 2 ? hi-- : lo++];
 
     bcsr_plan *plan = new (std::nothrow) bcsr_plan();
     if (!plan) return BCSR_ERR_OUT_OF_MEMORY;
@@ -259,7 +263,8 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     const size_t off_gp = align_up(off_idx + sizeof(int32_t) * std::max<int64_t>(nnzb, 1), A);
     const size_t off_gc = align_up(off_gp + sizeof(int32_t) * (dev_rows + 1), A);
     const size_t off_ro = align_up(off_gc + sizeof(int32_t) * std::max<size_t>(grp_col.size(), 4), A);
-    const size_t off_as = align_up(off_ro + sizeof(int32_t) * std::max(dev_rows, 1), A);
+    const size_t off_mo = align_up(off_ro + sizeof(int32_t) * std::max(dev_rows, 1), A);
+    const size_t off_as = align_up(off_mo + sizeof(int32_t) * std::max(dev_rows, 1), A);
     const size_t off_v = align_up(off_as + sizeof(uint32_t) * std::max<size_t>(ascale.size(), 1), A);
     const size_t off_pp = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
     const size_t off_pr = align_up(off_pp + sizeof(int32_t) * panel_ptr.size(), A);
@@ -279,6 +284,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     plan->d_grp_ptr = reinterpret_cast<int32_t *>(base + off_gp);
     plan->d_grp_col = reinterpret_cast<int32_t *>(base + off_gc);
     plan->d_row_order = reinterpret_cast<int32_t *>(base + off_ro);
+    plan->d_mix_order = reinterpret_cast<int32_t *>(base + off_mo);
     plan->d_ascale = fp8 ? reinterpret_cast<uint32_t *>(base + off_as) : nullptr;
     plan->d_vals = base + off_v;
     struct Copy {
@@ -290,6 +296,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
                   {plan->d_grp_ptr, grp_ptr.data(), sizeof(int32_t) * (dev_rows + 1)},
                   {plan->d_grp_col, grp_col.data(), sizeof(int32_t) * grp_col.size()},
                   {plan->d_row_order, row_order.data(), sizeof(int32_t) * dev_rows},
+                  {plan->d_mix_order, mix_order.data(), sizeof(int32_t) * dev_rows},
                   {base + off_as, ascale.data(), sizeof(uint32_t) * ascale.size()},
                   {plan->d_vals, vals.data(), vals.size()},
                   {base + off_pp, panel_ptr.data(), sizeof(int32_t) * panel_ptr.size()},
@@ -467,19 +474,23 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         if (plan->rs == 4 || (ls.n_tiles <= 2 && tile_n <= 128 && density >= 0.15))
             map = 2;
         else
-            map = xcd_ok ? 1 : 0;
+            map = xcd_ok ? 1 : 3;
     }
     if (map == 1 && !xcd_ok) return BCSR_ERR_INVALID_ARG;
-    if (map < 0 || map > 2) return BCSR_ERR_INVALID_ARG;
+    if (map < 0 || map > 3) return BCSR_ERR_INVALID_ARG;
     ls.map_mode = map;
     const int wpb = cfg && cfg->waves_per_block > 0 ? cfg->waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return BCSR_ERR_INVALID_ARG;
+    if (static_cast<int64_t>(ls.n_tiles) * plan->dev_rows + 8 > INT32_MAX) return BCSR_ERR_UNSUPPORTED;  // work items
     ls.wpb = wpb;
     const int rows = plan->dev_rows;
     if (map == 0) {
         ls.grid = (ls.n_tiles * rows + wpb - 1) / wpb;
     } else if (map == 2) {
         const int per_xcd = (rows + 7) / 8 * ls.n_tiles;  // items of one XCD
+        ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
+    } else if (map == 3) {
+        const int per_xcd = static_cast<int>((static_cast<int64_t>(ls.n_tiles) * rows + 7) / 8);
         ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
     } else if (ls.n_tiles >= 8) {
         const int per_xcd = ls.n_tiles / 8 * rows;  // items of one XCD

@@ -78,7 +78,7 @@ int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8
     a.grp_col = plan->d_grp_col;
     a.vals = static_cast<const u32x4 *>(plan->d_vals);
     a.ascale = plan->d_ascale;
-    a.row_order = plan->d_row_order;
+    a.row_order = ls.map_mode == 3 ? plan->d_mix_order : plan->d_row_order;
     a.B8 = B8;
     a.b8scale = bscale;
     a.C = C;

@@ -145,7 +145,7 @@ int launch_pairs(const bcsr_plan *plan, const LaunchShape &ls, const void *B, fl
     a.grp_ptr = plan->d_grp_ptr;
     a.grp_col = plan->d_grp_col;
     a.vals = static_cast<const u32x4 *>(plan->d_vals);
-    a.row_order = plan->d_row_order;
+    a.row_order = ls.map_mode == 3 ? plan->d_mix_order : plan->d_row_order;
     a.B = static_cast<const _Float16 *>(B);
     a.C = C;
     a.N = N;

@@ -168,7 +168,7 @@ def test_precision_uniform_values_target_shape_rows(mode):
     assert worst <= {"bf16": 2e-2, "hilo": 5e-5, "fp8": 2e-2}[mode]
 
 
-@pytest.mark.parametrize("xcd_map", [0, 1, 2])
+@pytest.mark.parametrize("xcd_map", [0, 1, 2, 3])
 @pytest.mark.parametrize("bh,N", [(16, 512), (64, 256), (32, 384)])
 def test_fp8_xcd_mappings_agree(bh, N, xcd_map):
     """The FP8 kernel under every work-to-XCD mapping gives the automatic

@@ -81,7 +81,7 @@ def launch_case(bh, long_rows):
 @pytest.mark.parametrize("wpb", [1, 4])
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("tile_n", [32, 64, 128, 256])
-@pytest.mark.parametrize("xcd_map", [0, 1, 2])
+@pytest.mark.parametrize("xcd_map", [0, 1, 2, 3])
 @pytest.mark.parametrize("bh", [16, 32, 64])
 def test_launch_configs(tile_n, xcd_map, bh, variant, wpb):
     case, C_cpu, Cabs = launch_case(bh, variant > 0)
@@ -256,6 +256,7 @@ def test_target_scale_sampled_rows_and_checksum():
     (32, 0.20, 128, "fp8", "bcsr_spmm_fp8<2, 4>"),                 # FP8, 32-row blocks, N = 128: 64 columns
     (16, 0.20, 512, "fp8", "bcsr_spmm_fp8<1, 16>"),                # FP8 dense: 256 columns
     (16, 0.05, 512, "fp8", "bcsr_spmm_fp8<1, 8>"),
+    (16, 0.05, 384, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),     # three tiles: chunked XCD mapping
 ])
 def test_automatic_choices(bh, density, N, prec, want):
     """The automatic launch picks the tile widths measured fastest (kernel
