@@ -257,15 +257,23 @@ def test_target_scale_sampled_rows_and_checksum():
     (16, 0.20, 512, "fp8", "bcsr_spmm_fp8<1, 16>"),                # FP8 dense: 256 columns
     (16, 0.05, 512, "fp8", "bcsr_spmm_fp8<1, 8>"),
     (16, 0.05, 384, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),     # three tiles: chunked XCD mapping
+    (16, 0.20, 1280, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),    # N > 1024: 128 columns
 ])
 def test_automatic_choices(bh, density, N, prec, want):
     """The automatic launch picks the tile widths measured fastest (kernel
-    names carry the block sub-tiles and 16-column groups per wave)."""
+    names carry the block sub-tiles and 16-column groups per wave), and a
+    product with few block-rows keeps 128-column tiles."""
     pattern = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}[bh]
-    M = K = 2048
+    M, K = 16384, 2048  # >= 1024 block-rows: the wide-tile rules apply
     h = mm.generate_bcsr(M, K, 1.0 - density, pattern, bh=bh, bw=16, seed=5)
     if prec == "f16":
         h = mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, bh, 16)
     plan = mm.Plan(h, prec)
     assert plan.kernel_name(N) == want
     plan.close()
+    if want.endswith("16, true, f16>"):
+        few = mm.generate_bcsr(2048, K, 1.0 - density, pattern, bh=bh, bw=16, seed=5)
+        few = mm.HostBCSR(few.hdr, few.idx, few.data.astype(np.float16), few.block_rows, few.block_cols, bh, 16)
+        plan = mm.Plan(few, prec)
+        assert plan.kernel_name(N) == "bcsr_spmm_pair<1, 8, true, f16>"
+        plan.close()
