@@ -411,11 +411,11 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         const double density = static_cast<double>(plan->info.nnzb) /
                                std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
         // 16-row blocks in the 16-bit modes at 512 <= N <= 1024 with >= 8 %
-        // blocks and >= 1024 block-rows: 256-column tiles (bench.py at
+        // blocks and >= 16384 rows: 256-column tiles (bench.py at
         // 16384^2, N = 512: 10 %: 81.3 vs 85.1 us, 20 %: 147.6 vs 152.9 us;
         // at 5 % 128 columns stay ahead, 49.5 vs 51.4 us).  Fewer rows give
         // too few work items for wider tiles (1280^2, N = 1280: 7.1 vs 5.5 us).
-        const bool wide_ok = plan->rs <= 2 && plan->dev_rows >= 1024 && N >= 512 && N <= 1024;
+        const bool wide_ok = plan->rs <= 2 && plan->dev_rows * plan->rs >= 1024 && N >= 512 && N <= 1024;
         if (!fp8 && plan->rs == 1 && plan->info.precision != BCSR_PREC_F16_HILO && wide_ok && density >= 0.08)
             tile_n = 256;
         if (fp8) {

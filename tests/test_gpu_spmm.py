@@ -256,6 +256,7 @@ def test_target_scale_sampled_rows_and_checksum():
     (32, 0.20, 128, "fp8", "bcsr_spmm_fp8<2, 4>"),                 # FP8, 32-row blocks, N = 128: 64 columns
     (16, 0.20, 512, "fp8", "bcsr_spmm_fp8<1, 16>"),                # FP8 dense: 256 columns
     (16, 0.05, 512, "fp8", "bcsr_spmm_fp8<1, 8>"),
+    (32, 0.20, 512, "fp8", "bcsr_spmm_fp8<2, 16>"),                # FP8 dense 32-row blocks: 256 columns
     (16, 0.05, 384, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),     # three tiles: chunked XCD mapping
     (16, 0.20, 1280, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),    # N > 1024: 128 columns
 ])
@@ -264,7 +265,7 @@ def test_automatic_choices(bh, density, N, prec, want):
     names carry the block sub-tiles and 16-column groups per wave), and a
     product with few block-rows keeps 128-column tiles."""
     pattern = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}[bh]
-    M, K = 16384, 2048  # >= 1024 block-rows: the wide-tile rules apply
+    M, K = 16384, 2048  # >= 16384 rows: the wide-tile rules apply
     h = mm.generate_bcsr(M, K, 1.0 - density, pattern, bh=bh, bw=16, seed=5)
     if prec == "f16":
         h = mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, bh, 16)
