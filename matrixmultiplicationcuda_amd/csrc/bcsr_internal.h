@@ -52,7 +52,7 @@ struct bcsr_plan {
     int32_t *d_grp_ptr = nullptr;
     int32_t *d_grp_col = nullptr;
     int32_t *d_row_order = nullptr;
-    int32_t *d_mix_order = nullptr;  // row_order taken longest, shortest, 2nd longest, ...
+    int32_t *d_mix_order = nullptr;  // row_order taken 32 longest, 32 shortest, next 32 longest, ...
     void *d_vals = nullptr;
     uint32_t *d_ascale = nullptr;
     // FP8: workspace holding the quantised B of the current call
@@ -137,9 +137,9 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
 // instead: XCD x takes the slots x, x + 8, ... with all their column tiles,
 // so a block-row's A is fetched into one XCD's L2 for every tile.  map_mode 3
 // is for tile counts that 8 does not divide: the items in tile-major order
-// (rows in the plan's mixed order, so that any run of them carries about the
-// average work) are cut into 8 equal runs, one per XCD, so each XCD serves
-// at most two column tiles.
+// (rows in the plan's mixed order, so that any run of them, and any CU's
+// share of it, carries about the average work) are cut into 8 equal runs,
+// one per XCD, so each XCD serves at most two column tiles.
 __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
     // threadIdx.x >> 6 is the same in every lane of a wave; readfirstlane
     // tells the compiler so, which keeps everything derived from it (row,

@@ -250,10 +250,11 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     for (int r = 0; r < dev_rows; ++r) row_order[r] = r;
     std::stable_sort(row_order.begin(), row_order.end(),
                      [&](int x, int y) { return grp_ptr[x + 1] - grp_ptr[x] > grp_ptr[y + 1] - grp_ptr[y]; });
-    // the same rows taken from both ends in turn: any run of consecutive
-    // entries then holds about the average work (chunked mapping)
+    // the same rows taken from both ends in turn, 32 at a time (the CUs of an
+    // XCD): any run of consecutive entries then holds about the average work,
+    // and so do the entries 32 apart that land on one CU (chunked mapping)
     std::vector<int32_t> mix_order(dev_rows);
-    for (int k = 0, lo = 0, hi = dev_rows - 1; k < dev_rows; ++k) mix_order[k] = row_order[k % 2 ? hi-- : lo++];
+    for (int k = 0, lo = 0, hi = dev_rows - 1; k < dev_rows; ++k) mix_order[k] = row_order[(k / 32) % 2 ? hi-- : lo++];
 
     bcsr_plan *plan = new (std::nothrow) bcsr_plan();
     if (!plan) return BCSR_ERR_OUT_OF_MEMORY;
