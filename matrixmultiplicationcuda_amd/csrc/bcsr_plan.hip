@@ -395,6 +395,9 @@ int entry(int bh, const int *hdr, const int *idx, const void *data, const void *
 
 int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls) {
     const bool fp8 = plan->info.precision == BCSR_PREC_FP8_E4M3;
+    // fraction of the bh x bw blocks that are stored
+    const double density = static_cast<double>(plan->info.nnzb) /
+                           std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
     int tile_n = cfg ? cfg->tile_n : 0;
     if (tile_n == 0 && cfg && cfg->variant == 3) tile_n = 128;  // the panel kernel's only width
     if (tile_n == 0) {
@@ -409,8 +412,6 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // fp16 N = 64: 4.6 vs 5.2 us (1 %), 11.9 vs 13.1 (5 %), 32.5 vs 37.1
         // (20 %); N = 128 and FP8: 2-14 % faster
         if (plan->rs == 4 && N <= 128) tile_n = 32;
-        const double density = static_cast<double>(plan->info.nnzb) /
-                               std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
         // 16-row blocks in the 16-bit modes at 512 <= N <= 1024 with >= 8 %
         // blocks and >= 16384 rows: 256-column tiles (bench.py at
         // 16384^2, N = 512: 10 %: 81.3 vs 85.1 us, 20 %: 147.6 vs 152.9 us;
@@ -471,9 +472,9 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // (N = 256: 1.15-1.34x, N = 512 at 20 %: 1.11x) and for dense
         // matrices with at most two tiles of <= 128 columns (20 %, N = 256:
         // 1.04-1.11x); tiles win elsewhere (16 x 16, 5 %, N = 512: 52 vs 84
-        // us; two 256-column tiles at 20 %: 143 vs 165 us).
-        const double density = static_cast<double>(plan->info.nnzb) /
-                               std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
+        // us; two 256-column tiles at 20 %: 143 vs 165 us).  Tile counts that
+        // 8 does not divide take the chunked runs (N = 384: 41.5 vs 59.1 us
+        // in the linear order).
         if (plan->rs == 4 || (ls.n_tiles <= 2 && tile_n <= 128 && density >= 0.15))
             map = 2;
         else
