@@ -91,7 +91,13 @@ int bcsr_plan_destroy(bcsr_plan *plan);
 int bcsr_plan_get_info(const bcsr_plan *plan, bcsr_plan_info *info);
 
 /* C (device, M x N fp32) = plan . B (device, K x N fp16).  N % 16 == 0,
- * B and C 16-byte aligned.  cfg may be NULL (automatic choice). */
+ * B and C 16-byte aligned.  cfg may be NULL (automatic choice).  Calls of
+ * the 16-bit modes only read the plan, so they may overlap on any streams.
+ * An FP8 plan quantises B into a workspace of its own (allocated by the
+ * first call, grown by a larger N, the outgrown ones kept until
+ * bcsr_plan_destroy), so the caller orders calls of one FP8 plan (one
+ * stream, or events); bcsr_quantize_b_e4m3 + bcsr_spmm_b8 take the caller's
+ * buffers instead. */
 int bcsr_spmm(const bcsr_plan *plan, const void *B, float *C, int32_t N, void *stream);
 int bcsr_spmm_cfg(const bcsr_plan *plan, const void *B, float *C, int32_t N,
                   const bcsr_launch_cfg *cfg, void *stream);

@@ -55,10 +55,14 @@ struct bcsr_plan {
     int32_t *d_mix_order = nullptr;  // row_order taken 32 longest, 32 shortest, next 32 longest, ...
     void *d_vals = nullptr;
     uint32_t *d_ascale = nullptr;
-    // FP8: workspace holding the quantised B of the current call
+    // FP8: workspace holding the quantised B of the current call.  A
+    // workspace outgrown by a larger N is kept (ws_retired) until the plan is
+    // destroyed: a graph that captured an earlier call still points at it,
+    // and freeing it would synchronise the device.
     std::mutex *ws_mu = nullptr;
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
+    std::vector<void *> ws_retired;
     // host copies of grp_ptr and row_order (work-item construction)
     std::vector<int32_t> h_grp_ptr, h_row_order;
     // panel schedule (16-row device blocks, fp16 / bf16 only; n_panels = 0

@@ -145,6 +145,11 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     const int64_t nnzb = hdr[block_rows];
     const int64_t blk_elems = static_cast<int64_t>(bh) * bw;
 
+    // group offsets are 32-bit on the device
+    int64_t total_groups = 0;
+    for (int R = 0; R < block_rows; ++R)
+        total_groups += (static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces + G - 1) / G * sub_rows;
+    if (total_groups > INT32_MAX) return BCSR_ERR_SHAPE;
     std::vector<int32_t> grp_ptr(dev_rows + 1, 0);
     for (int R = 0; R < block_rows; ++R) {
         const int64_t nblk = static_cast<int64_t>(hdr[R + 1] - hdr[R]) * pieces;
@@ -557,8 +562,9 @@ int bcsr_plan_create_f32(const int32_t *hdr, const int32_t *idx, const float *da
 int bcsr_plan_destroy(bcsr_plan *plan) {
     if (!plan) return BCSR_OK;
     int rc = hip_status(hipFree(plan->dev_base));
-    if (plan->d_ws) {
-        int rc2 = hip_status(hipFree(plan->d_ws));
+    if (plan->d_ws) plan->ws_retired.push_back(plan->d_ws);
+    for (void *ws : plan->ws_retired) {
+        int rc2 = hip_status(hipFree(ws));
         if (rc == BCSR_OK) rc = rc2;
     }
     delete plan->ws_mu;
@@ -648,9 +654,12 @@ int bcsr_copy_to_device(const int32_t *hdr, const int32_t *idx, const void *data
     bcsr_plan *plan = nullptr;
     int rc = bcsr_plan_create(hdr, idx, data_f16, block_rows, block_cols, bh, bw, &plan);
     if (rc != BCSR_OK) return rc;
-    {
+    try {
         std::lock_guard<std::mutex> lk(g_reg_mu);
         registry()[plan->d_hdr] = plan;
+    } catch (const std::bad_alloc &) {
+        (void)bcsr_plan_destroy(plan);
+        return BCSR_ERR_OUT_OF_MEMORY;
     }
     *gpu_hdr = plan->d_hdr;
     *gpu_idx = plan->d_idx;

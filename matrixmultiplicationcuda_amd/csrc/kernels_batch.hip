@@ -17,6 +17,7 @@
 // captured graph holds kernel nodes only.
 #include <algorithm>
 #include <cstdio>
+#include <new>
 #include <numeric>
 
 #include "fp8_device.h"
@@ -196,9 +197,8 @@ extern "C" int bcsr_batch_tile_supported(int32_t precision, int32_t dev_bh, int3
     return batch_shape(precision, dev_bh / 16, tile_n / 16) ? 1 : 0;
 }
 
-extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C,
-                                 const int32_t *N, const int32_t *tile_n, int32_t count,
-                                 const bcsr_batch_cfg *cfg, bcsr_batch **out) {
+static int batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C, const int32_t *N,
+                        const int32_t *tile_n, int32_t count, const bcsr_batch_cfg *cfg, bcsr_batch **out) {
     if (!out || count < 0 || (count > 0 && (!plans || !B || !C || !N || !plans[0]))) return BCSR_ERR_INVALID_ARG;
     *out = nullptr;
     int dev = 0;
@@ -388,6 +388,18 @@ extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *cons
     b->tiles = std::move(tiles);
     *out = b;
     return BCSR_OK;
+}
+
+// the host-side work lists are std::vectors: an allocation failure is a
+// status, never an exception through the C ABI
+extern "C" int bcsr_batch_create(const bcsr_plan *const *plans, const void *const *B, float *const *C,
+                                 const int32_t *N, const int32_t *tile_n, int32_t count,
+                                 const bcsr_batch_cfg *cfg, bcsr_batch **out) {
+    try {
+        return batch_create(plans, B, C, N, tile_n, count, cfg, out);
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
 }
 
 extern "C" int bcsr_batch_run(bcsr_batch *batch, void *stream) {

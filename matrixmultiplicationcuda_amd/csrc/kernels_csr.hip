@@ -19,6 +19,7 @@
 #include <rocblas/rocblas.h>
 
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "bcsr_internal.h"
@@ -108,7 +109,11 @@ int blas_handle(rocblas_handle &h) {
     int rc = hip_status(hipGetDevice(&dev));
     if (rc != BCSR_OK) return rc;
     std::lock_guard<std::mutex> lk(g_blas_mu);
-    if (static_cast<int>(g_blas.size()) <= dev) g_blas.resize(dev + 1, nullptr);
+    try {
+        if (static_cast<int>(g_blas.size()) <= dev) g_blas.resize(dev + 1, nullptr);
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
     if (!g_blas[dev] && rocblas_create_handle(&g_blas[dev]) != rocblas_status_success) {
         g_blas[dev] = nullptr;
         return BCSR_ERR_UNSUPPORTED;

@@ -3,6 +3,7 @@
 // product kernels.  The device code is in fp8_device.h.
 #include <algorithm>
 #include <cstdio>
+#include <new>
 
 #include "fp8_device.h"
 
@@ -106,15 +107,21 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     const size_t need = off_p + work_bytes;
     std::lock_guard<std::mutex> lk(*plan->ws_mu);
     if (plan->ws_bytes < need) {
-        if (plan->d_ws) {
-            int rc = hip_status(hipFree(plan->d_ws));
-            plan->d_ws = nullptr;
-            plan->ws_bytes = 0;
-            if (rc != BCSR_OK) return rc;
-        }
-        int rc = hip_status(hipMalloc(&plan->d_ws, need));
+        // grow at least 2x, so a plan called with rising N keeps few buffers
+        const size_t bytes = std::max(need, 2 * plan->ws_bytes);
+        void *ws = nullptr;
+        int rc = hip_status(hipMalloc(&ws, bytes));
         if (rc != BCSR_OK) return rc;
-        plan->ws_bytes = need;
+        if (plan->d_ws) {
+            try {
+                plan->ws_retired.push_back(plan->d_ws);
+            } catch (const std::bad_alloc &) {
+                (void)hipFree(ws);
+                return BCSR_ERR_OUT_OF_MEMORY;
+            }
+        }
+        plan->d_ws = ws;
+        plan->ws_bytes = bytes;
     }
     uint8_t *B8 = static_cast<uint8_t *>(plan->d_ws);
     uint8_t *bscale = B8 + off_s;

@@ -13,6 +13,7 @@
 // summation order: the backward pass is deterministic.
 #include <cmath>
 #include <cstdint>
+#include <new>
 #include <random>
 #include <vector>
 
@@ -255,7 +256,13 @@ extern "C" int bcsr_cm_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uin
     cm->ob3 = cm->oW3 + n_out * n_hidden;
     cm->n_params = cm->ob3 + n_out;
     // initial weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases zero
-    std::vector<float> h(cm->n_params, 0.0f);
+    std::vector<float> h;
+    try {
+        h.assign(cm->n_params, 0.0f);
+    } catch (const std::bad_alloc &) {
+        delete cm;
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
     std::mt19937 rng(seed);
     auto fill = [&](int off, int rows, int fan_in) {
         std::uniform_real_distribution<float> u(-1.0f / std::sqrt(static_cast<float>(fan_in)),

@@ -196,3 +196,20 @@ def test_fp8_xcd_mappings_agree(bh, N, xcd_map):
         mm.synchronize()
         assert np.array_equal(C1.numpy(), C2.numpy())
     plan.close()
+
+
+def test_fp8_workspace_growth_keeps_results():
+    """One FP8 plan called with N = 64, 512 (its workspace grows), then 64
+    again gives, each time, the C of a fresh plan called once at that N."""
+    M, K = 512, 1024
+    h32, _, _ = case(M, K, 16, 16)
+    plan = mm.Plan(h32, "fp8")
+    for N in (64, 512, 64, 256):
+        B16 = mm.generate(K, N, 0.0, "random", seed=N)
+        log(f"fp8 workspace N={N}")
+        C = plan.matmul(B16)
+        fresh = mm.Plan(h32, "fp8")
+        C_ref = fresh.matmul(B16)
+        fresh.close()
+        assert np.array_equal(C, C_ref), N
+    plan.close()
