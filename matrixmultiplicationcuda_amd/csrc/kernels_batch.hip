@@ -85,8 +85,14 @@ __global__ void batch_reset(int *counter) {
 }
 
 // One wave per workgroup; every wave leaves once the counter passes n_items.
+// The FP8 form is held to 256 VGPRs, so two of its waves share each SIMD:
+// unbounded it takes 268 (one wave per SIMD), bounded 254 with no spills, and
+// it runs the 96-problem workload in 0.477 against 0.497 ms on MI355X.  The
+// 16-bit forms already fit two waves (170-174 VGPRs); holding them to three
+// (168) spills and is slower (0.564 against 0.554 ms for fp16).
 template <int PREC>
-__global__ __launch_bounds__(64) void bcsr_batch_persistent(const BatchMat *__restrict__ mats,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC == BPREC_FP8 ? 2 : 1))) void
+bcsr_batch_persistent(const BatchMat *__restrict__ mats,
                                                             const int4 *__restrict__ items, int n_items,
                                                             int *counter) {
     for (;;) {
