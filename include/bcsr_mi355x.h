@@ -316,6 +316,20 @@ int bcsr_event_create(void **event);
 int bcsr_event_destroy(void *event);
 int bcsr_event_record(void *event, void *stream);
 int bcsr_event_elapsed_ms(void *start, void *end, float *ms);
+/* HIP graphs of this library's launches (no reference counterpart: the
+ * reference launches every kernel from the host).  bcsr_graph_begin starts
+ * capturing `stream` (not NULL; thread-local capture mode); the library
+ * calls that follow on it are recorded, not run; bcsr_graph_end stops and
+ * instantiates, giving an executable graph that bcsr_graph_launch replays on
+ * any stream.  The library's launches record kernel nodes only (no memset
+ * nodes).  An FP8 plan call allocates its workspace when it needs a larger
+ * one than the plan holds, which a capture cannot record
+ * (BCSR_ERR_UNSUPPORTED): call the plan once at that N before capturing.
+ * The captured workspace stays valid until bcsr_plan_destroy. */
+int bcsr_graph_begin(void *stream);
+int bcsr_graph_end(void *stream, void **graph_exec);
+int bcsr_graph_launch(void *graph_exec, void *stream);
+int bcsr_graph_destroy(void *graph_exec);
 
 /* ---- host-side generator and formats ----------------------------------------
  * mg::generate_matrix<T> (matrix_generator.cpp:28-140) written row-major as

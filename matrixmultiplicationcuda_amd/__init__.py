@@ -201,6 +201,10 @@ def _declare(lib):
     f("bcsr_event_destroy", i, _P)
     f("bcsr_event_record", i, _P, _P)
     f("bcsr_event_elapsed_ms", i, _P, _P, ctypes.POINTER(ctypes.c_float))
+    f("bcsr_graph_begin", i, _P)
+    f("bcsr_graph_end", i, _P, ctypes.POINTER(_P))
+    f("bcsr_graph_launch", i, _P, _P)
+    f("bcsr_graph_destroy", i, _P)
 
 
 def lib():
@@ -432,6 +436,39 @@ class Event:
     def destroy(self):
         if self.handle:
             check(lib().bcsr_event_destroy(self.handle), "hipEventDestroy")
+            self.handle = ctypes.c_void_p()
+
+
+class Graph:
+    """A HIP graph of this library's launches, captured from a Stream:
+
+        g = Graph.capture(stream, lambda: plan.spmm_ptr(B, C, N, stream))
+        g.launch(stream)   # replays the recorded kernels
+    """
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def capture(cls, stream: "Stream", record) -> "Graph":
+        check(lib().bcsr_graph_begin(stream_handle(stream)), "bcsr_graph_begin")
+        h = ctypes.c_void_p()
+        try:
+            record()
+        except BaseException:
+            lib().bcsr_graph_end(stream_handle(stream), ctypes.byref(h))  # stop capturing
+            if h:
+                lib().bcsr_graph_destroy(h)
+            raise
+        check(lib().bcsr_graph_end(stream_handle(stream), ctypes.byref(h)), "bcsr_graph_end")
+        return cls(h)
+
+    def launch(self, stream=None):
+        check(lib().bcsr_graph_launch(self.handle, stream_handle(stream)), "bcsr_graph_launch")
+
+    def destroy(self):
+        if self.handle:
+            check(lib().bcsr_graph_destroy(self.handle), "bcsr_graph_destroy")
             self.handle = ctypes.c_void_p()
 
 

@@ -112,5 +112,44 @@ int bcsr_event_elapsed_ms(void *start, void *end, float *ms) {
     return hip_status(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
 }
 
+// Stream capture into a HIP graph: the launches enqueued on `stream` between
+// begin and end are recorded, not run; the instantiated graph replays them.
+// Thread-local capture mode, so other threads' HIP calls are unaffected.
+int bcsr_graph_begin(void *stream) {
+    if (!stream) return BCSR_ERR_INVALID_ARG;  // the null stream cannot be captured
+    return hip_status(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+}
+
+int bcsr_graph_end(void *stream, void **graph_exec) {
+    if (!stream || !graph_exec) return BCSR_ERR_INVALID_ARG;
+    *graph_exec = nullptr;
+    hipGraph_t g = nullptr;
+    int rc = hip_status(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g));
+    if (rc != BCSR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    hipGraphExec_t e = nullptr;
+    rc = hip_status(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    const int rc_destroy = hip_status(hipGraphDestroy(g));  // the executable graph keeps its own copy
+    if (rc != BCSR_OK) return rc;
+    if (rc_destroy != BCSR_OK) {
+        (void)hipGraphExecDestroy(e);
+        return rc_destroy;
+    }
+    *graph_exec = e;
+    return BCSR_OK;
+}
+
+int bcsr_graph_launch(void *graph_exec, void *stream) {
+    if (!graph_exec) return BCSR_ERR_INVALID_ARG;
+    return hip_status(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), static_cast<hipStream_t>(stream)));
+}
+
+int bcsr_graph_destroy(void *graph_exec) {
+    if (!graph_exec) return BCSR_OK;
+    return hip_status(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+}
+
 }  // extern "C"
 

@@ -107,6 +107,11 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     const size_t need = off_p + work_bytes;
     std::lock_guard<std::mutex> lk(*plan->ws_mu);
     if (plan->ws_bytes < need) {
+        // an allocation cannot be recorded into a graph under capture
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        int rc0 = hip_status(hipStreamIsCapturing(stream, &cap));
+        if (rc0 != BCSR_OK) return rc0;
+        if (cap != hipStreamCaptureStatusNone) return BCSR_ERR_UNSUPPORTED;
         // grow at least 2x, so a plan called with rising N keeps few buffers
         const size_t bytes = std::max(need, 2 * plan->ws_bytes);
         void *ws = nullptr;
