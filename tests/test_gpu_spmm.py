@@ -15,7 +15,7 @@ import pytest
 
 import matrixmultiplicationcuda_amd as mm
 import oracle
-from helpers import assert_close, cpu_reference, make_case
+from helpers import Case, assert_close, cpu_reference, make_case
 
 pytestmark = pytest.mark.gpu
 
@@ -278,3 +278,47 @@ def test_automatic_choices(bh, density, N, prec, want):
         plan = mm.Plan(few, prec)
         assert plan.kernel_name(N) == "bcsr_spmm_pair<1, 8, true, f16>"
         plan.close()
+
+
+def test_b_beyond_2gib_uses_64bit_addressing():
+    """B of 1048704 x 1024 fp16 (2.0 GiB + 800 KiB): 32-bit buffer offsets no
+    longer reach it, so the automatic choice takes the 64-bit global-load
+    kernel.  Four block-rows hold blocks at block-columns near both ends of K
+    (the last ones lie beyond byte offset 2^31 of B); the product is checked
+    against the oracle on the compacted slabs those blocks use."""
+    K, N, M = 1048704, 1024, 64
+    bc = K // 16
+    rng = np.random.default_rng(9)
+    cols = [sorted({0, r + 1, 40000 + r, bc - 1 - r}) for r in range(M // 16)]
+    hdr = np.zeros(M // 16 + 1, np.int32)
+    hdr[1:] = np.cumsum([len(c) for c in cols])
+    idx = np.concatenate([np.asarray(c, np.int32) for c in cols])
+    data = rng.uniform(-1, 1, (len(idx), 16, 16)).astype(np.float16)
+    h = mm.HostBCSR(hdr, idx, data, M // 16, bc, 16, 16)
+    used = np.unique(idx)
+    B = np.zeros((K, N), np.float16)
+    slabs = rng.uniform(-1, 1, (len(used), 16, N)).astype(np.float16)
+    for s, c in zip(slabs, used):
+        B[16 * c:16 * c + 16] = s
+    assert K * N * 2 > 2**31 - 1
+    print(f"K={K} N={N} B bytes={K * N * 2} nnzb={len(idx)}", flush=True)
+    dB = mm.DeviceArray.from_numpy(B)
+    del B
+    plan = mm.Plan(h, "f16")
+    name = plan.kernel_name(N)
+    assert name.startswith("bcsr_spmm_pair_global"), name
+    C = mm.DeviceArray((M, N), np.float32)
+    C.fill_bytes(0xFF)
+    plan.spmm_ptr(dB.ptr, C.ptr, N)
+    mm.synchronize()
+    got = C.numpy()
+    # the same product on the compacted slabs
+    remap = {int(c): i for i, c in enumerate(used)}
+    small = Case(np.zeros((M, 16 * len(used)), np.float16), slabs.reshape(-1, N),
+                 mm.HostBCSR(hdr, np.asarray([remap[int(c)] for c in idx], np.int32), data, M // 16,
+                             len(used), 16, 16), 16, 16)
+    C_cpu, Cabs = cpu_reference(small)
+    assert_close(got, C_cpu, Cabs, what="B > 2 GiB")
+    for a in (C, dB):
+        a.free()
+    plan.close()
