@@ -420,6 +420,12 @@ def main():
             if sum(times) > 20.0:
                 break
         tc = float(np.median(times))
+        # the same loop on one core (one run)
+        c0 = time.perf_counter()
+        C_one = oracle.bcsr_spmm_f32(h.hdr, h.idx, data16, bh, bw, Bf, threads=1)
+        t1 = time.perf_counter() - c0
+        assert np.array_equal(C_one, C_cpu), "the oracle's product depends on its thread count"
+        del C_one
         C64, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data.astype(np.float32), bh, bw, Bf,
                                          threads=threads)
         err = np.abs(C_gpu.astype(np.float64) - C_cpu)
@@ -445,6 +451,8 @@ def main():
                          f"loop over {threads} threads, median of {len(times)} runs "
                          f"({tc:.3f} s each)",
                "seconds": round(tc, 4),
+               "single_core": {"value": round(flops / t1 / 1e12, 5), "unit": "TFLOPS", "cores": 1,
+                               "seconds": round(t1, 4)},
                "gpu_vs_cpu": {"max_err_over_absAabsB": rel, "bound": bound,
                               "exact_zero_where_absAabsB_zero": exact_zero,
                               "ok": rel <= bound and exact_zero},
