@@ -73,3 +73,42 @@ def test_random_structures_and_launches(seed):
             a.free()
         plan.close()
     assert ran >= 12, (ran, refused)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_structures_precision_modes(seed):
+    """bf16, hi+lo and FP8 plans of random structures (fp32 block values,
+    automatic launch) against the exact product of the values each mode
+    stores: 1e-4 |Aq||Bq| per entry for the 16-bit modes (fp32 accumulation
+    only), 2^-11 for FP8 (its scaled MFMA does not add in IEEE fp32)."""
+    import oracle
+
+    rng = np.random.default_rng(2000 + seed)
+    for case_no in range(5):
+        bh = int(rng.choice([16, 32, 48, 64]))
+        bw = int(rng.choice([16, 32]))
+        br, bc = int(rng.integers(1, 9)), int(rng.integers(1, 17))
+        rows = [np.sort(rng.choice(bc, size=int(rng.binomial(bc, 0.5)), replace=False)).astype(np.int32)
+                for _ in range(br)]
+        hdr = np.zeros(br + 1, np.int32)
+        hdr[1:] = np.cumsum([len(r) for r in rows])
+        idx = np.concatenate(rows) if hdr[-1] else np.zeros(0, np.int32)
+        data32 = (rng.uniform(-1, 1, (int(hdr[-1]), bh, bw)) *
+                  np.exp(rng.uniform(-4, 4, (int(hdr[-1]), 1, 1)))).astype(np.float32)
+        h32 = mm.HostBCSR(hdr, idx, data32, br, bc, bh, bw)
+        M, K = h32.shape
+        N = 16 * int(rng.integers(1, 25))
+        B16 = rng.uniform(-1, 1, (K, N)).astype(np.float16)
+        for mode in ("bf16", "hilo", "fp8"):
+            what = f"seed={seed} case={case_no} mode={mode} bh={bh} bw={bw} M={M} K={K} N={N} nnzb={h32.nnzb}"
+            print(what, flush=True)
+            plan = mm.Plan(h32, mode)
+            C = plan.matmul(B16).astype(np.float64)
+            plan.close()
+            Aq = np.asarray(oracle.dequantized_blocks(mode, data32, bh, bw, hdr, idx), dtype=np.float64)
+            Aq32 = Aq.astype(np.float32)
+            assert np.array_equal(Aq32.astype(np.float64), Aq), what  # every stored value is an fp32
+            Bq = oracle.quantized_B(mode, B16.view(np.uint16))
+            Cq, Cqabs = oracle.bcsr_spmm_f64(hdr, idx, Aq32, bh, bw, Bq, threads=8)
+            tight = 2.0**-11 if mode == "fp8" else 1e-4
+            assert np.all(np.abs(C - Cq) <= tight * Cqabs + 1e-30), what
