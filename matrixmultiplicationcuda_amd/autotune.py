@@ -89,16 +89,25 @@ def record(key, feats, N, precision, configs, times, auto_ms):
             "auto_ms": float(auto_ms)}
 
 
+def batch_variant(precision_id, dev_bh, tile):
+    """The kernel variant the batched persistent kernel runs for a device
+    block height and tile width: the prefetching one (1) where its two load
+    sets fit, else the plain one (0), as batch_prefetch in kernels_batch.hip.
+    FP8 has one kernel, timed as variant 1."""
+    h = 2 if precision_id == 3 else 1
+    return 1 if precision_id == 2 or (dev_bh // 16) * (tile // 16) * h <= 16 else 0
+
+
 def batch_tiles_from_times(plans, Ns, times, configs, precision_id):
     """Each problem's fastest tile width among those the persistent kernel
-    supports (its timings of the pipelined variant, else any variant)."""
+    supports, judged by the timings of the variant the batch runs there."""
     tiles = []
     for p, N, t in zip(plans, Ns, times):
         best, best_ms = None, np.inf
         dev_bh = p.info.dev_bh
         for c, ms in zip(configs, t):
             tile, variant = c
-            if not np.isfinite(ms) or variant == 0:
+            if not np.isfinite(ms) or variant != batch_variant(precision_id, dev_bh, tile):
                 continue
             if not lib().bcsr_batch_tile_supported(precision_id, dev_bh, tile):
                 continue
@@ -151,5 +160,5 @@ def precision_id(precision):
 
 
 __all__ = ["CONFIGS_16", "CONFIGS_FP8", "TimingDB", "time_configs", "time_auto", "tune_batch",
-           "batch_tiles_from_times", "tune_batch_policies", "configs_for", "cfg_of",
+           "batch_variant", "batch_tiles_from_times", "tune_batch_policies", "configs_for", "cfg_of",
            "supported_mask", "record", "check"]

@@ -43,13 +43,22 @@ struct QuantJob {
     int wg0;  // first workgroup index of this job (prefix over slabs * chunks)
 };
 
-// Block shapes the persistent kernel is compiled for: the kernel's register
-// allocation is the maximum over its branches, so the large ones (64-row
-// blocks with 128-column tiles, hi+lo beyond 8 accumulator tiles) are left to
-// the per-matrix kernels.
+// Block shapes the persistent kernel is compiled for.  Its register
+// allocation is the maximum over its branches, so shapes whose work function
+// would push a form past 256 VGPRs (one wave per SIMD) are left to the
+// per-matrix kernels: 256-column tiles, and for FP8 and hi+lo the 64-row,
+// 128-column tile.  fp16 and bf16 keep that tile through the plain work
+// function (208 VGPRs; the 96-problem workload takes 0.541 against 0.555 ms
+// for fp16 and 0.542 against 0.558 ms for bf16 with 64-column tiles).
 __host__ __device__ constexpr bool batch_shape(int prec, int rs, int epl) {
     const int h = prec == BPREC_HILO ? 2 : 1;
+    if (rs == 4 && epl == 8) return prec == BPREC_F16 || prec == BPREC_BF16;
     return (rs == 1 || rs == 2 || rs == 4) && (epl == 2 || epl == 4 || epl == 8) && rs * epl * h <= 16;
+}
+
+// The prefetching work function where its two load sets fit, else the plain one.
+__host__ __device__ constexpr bool batch_prefetch(int prec, int rs, int epl) {
+    return rs * epl * (prec == BPREC_HILO ? 2 : 1) <= 16;
 }
 
 template <int PREC, int RS, int EPL>
@@ -60,7 +69,7 @@ __device__ __forceinline__ void run_rows(const SpmmArgs &a, int tile, int s0, in
             if constexpr (PREC == BPREC_FP8)
                 fp8dev::fp8_work<RS, EPL>(a, br, tile);
             else
-                pairdev::pair_work<RS, EPL, true, PREC>(a, br, tile);
+                pairdev::pair_work<RS, EPL, batch_prefetch(PREC, RS, EPL), PREC>(a, br, tile);
         }
     }
 }
@@ -76,6 +85,7 @@ __device__ __forceinline__ void run_item(const SpmmArgs &a, int key, int tile, i
         case 0x28: run_rows<PREC, 2, 8>(a, tile, s0, s1); break;
         case 0x42: run_rows<PREC, 4, 2>(a, tile, s0, s1); break;
         case 0x44: run_rows<PREC, 4, 4>(a, tile, s0, s1); break;
+        case 0x48: run_rows<PREC, 4, 8>(a, tile, s0, s1); break;
         default: break;
     }
 }
@@ -88,8 +98,9 @@ __global__ void batch_reset(int *counter) {
 // The FP8 form is held to 256 VGPRs, so two of its waves share each SIMD:
 // unbounded it takes 268 (one wave per SIMD), bounded 254 with no spills, and
 // it runs the 96-problem workload in 0.477 against 0.497 ms on MI355X.  The
-// 16-bit forms already fit two waves (170-174 VGPRs); holding them to three
-// (168) spills and is slower (0.564 against 0.554 ms for fp16).
+// 16-bit forms already fit two waves (174-208 VGPRs); holding fp16 to three
+// (168, before the 64 x 128 shape was added) spilled and was slower (0.564
+// against 0.554 ms).
 template <int PREC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC == BPREC_FP8 ? 2 : 1))) void
 bcsr_batch_persistent(const BatchMat *__restrict__ mats,
@@ -192,6 +203,7 @@ int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
 
 int auto_batch_tile(const bcsr_plan *p, int N) {
     int tile = N >= 128 ? 128 : (N >= 64 ? 64 : 32);
+    if (p->rs == 4 && N < 512) tile = std::min(tile, 64);  // as the per-matrix rule
     while (tile > 32 && !batch_shape(p->info.precision, p->rs, tile / 16)) tile /= 2;
     return tile;
 }

@@ -252,9 +252,9 @@ class Dispatcher:
 
     def batch_tiles(self, plans, hosts, Ns):
         """Per-problem tile widths for a batch: the predicted-fastest
-        configuration among those the persistent kernel runs (it always uses
-        the pipelined variant of the 16-bit kernels)."""
-        from . import PRECISIONS
+        configuration among those the persistent kernel runs (the variant
+        autotune.batch_variant names for each tile width)."""
+        from . import PRECISIONS, autotune
         prec = PRECISIONS[self.precision]
         pred = self.predict(list(zip(hosts, Ns)))
         tiles = []
@@ -263,7 +263,8 @@ class Dispatcher:
             best, best_v = 0, np.inf
             for c, v in zip(self.configs, row):
                 tile, variant = c
-                if variant == 0 or not lib().bcsr_batch_tile_supported(prec, dev_bh, tile):
+                if variant != autotune.batch_variant(prec, dev_bh, tile) or \
+                        not lib().bcsr_batch_tile_supported(prec, dev_bh, tile):
                     continue
                 if v < best_v:
                     best, best_v = tile, v

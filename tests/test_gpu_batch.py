@@ -2,7 +2,8 @@
 
 The persistent kernel runs the per-matrix kernels' work functions over a
 device work queue, so each problem's C must equal, bit for bit, the
-per-matrix launch with the same tile width (pipelined variant); the fp16
+per-matrix launch with the same tile width (automatic variant: the
+prefetching kernel where it fits, as in the batch); the fp16
 batch is also checked against the oracle's CPU product (1e-3 of |A||B|).
 Graph replays are checked one by one with eager work in between.
 """
@@ -45,7 +46,9 @@ def setup(problems, precision):
 def per_matrix(plans, Bs, tiles):
     out = []
     for p, B, t in zip(plans, Bs, tiles):
-        cfg = mm.make_cfg(tile_n=int(t), variant=1)
+        # automatic variant: the prefetching kernel where it fits, else the
+        # plain one (64-row blocks at 128 columns), as the batch runs them
+        cfg = mm.make_cfg(tile_n=int(t), variant=-1)
         out.append(p.matmul(B.numpy(), cfg=cfg))
     return out
 
