@@ -91,7 +91,8 @@ int bcsr_plan_destroy(bcsr_plan *plan);
 int bcsr_plan_get_info(const bcsr_plan *plan, bcsr_plan_info *info);
 
 /* C (device, M x N fp32) = plan . B (device, K x N fp16).  N % 16 == 0,
- * B and C 16-byte aligned.  cfg may be NULL (automatic choice).  Calls of
+ * B and C 16-byte aligned.  cfg may be NULL (automatic choice).  The plan's
+ * device must be the current one (BCSR_ERR_INVALID_ARG otherwise).  Calls of
  * the 16-bit modes only read the plan, so they may overlap on any streams.
  * An FP8 plan quantises B into a workspace of its own (allocated by the
  * first call, grown by a larger N, the outgrown ones kept until

@@ -347,6 +347,11 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
 int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N) {
     if (!plan) return BCSR_ERR_INVALID_ARG;
     if (N < 0) return BCSR_ERR_INVALID_ARG;
+    // the plan's arrays live on the device it was built on
+    int dev = 0;
+    const int rc = hip_status(hipGetDevice(&dev));
+    if (rc != BCSR_OK) return rc;
+    if (dev != plan->device) return BCSR_ERR_INVALID_ARG;
     if (N % 16 != 0) return BCSR_ERR_SHAPE;
     if (N == 0 || plan->info.rows == 0) return BCSR_OK;
     if (!B || !C) return BCSR_ERR_INVALID_ARG;
