@@ -142,7 +142,10 @@ int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cf
  * (counter reset + persistent kernel; FP8 first quantises every B);
  * bcsr_batch_run_graph captures that sequence into a hipGraph on its first
  * call and replays it (stream must not be NULL).  Results equal the
- * per-matrix launches bit for bit. */
+ * per-matrix launches with the same tile widths bit for bit.  The batch keeps
+ * the plans' device arrays and the B_i, C_i pointers: plans, B and C must
+ * outlive it.  Runs of one batch share its queue counter (and FP8
+ * workspaces), so they must be ordered (one stream, or events). */
 typedef struct bcsr_batch bcsr_batch;
 typedef struct {
     int32_t waves_per_cu; /* persistent waves per CU: 0 = 8 */
