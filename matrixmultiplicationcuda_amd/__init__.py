@@ -461,7 +461,9 @@ class Graph:
                 lib().bcsr_graph_destroy(h)
             raise
         check(lib().bcsr_graph_end(stream_handle(stream), ctypes.byref(h)), "bcsr_graph_end")
-        return cls(h)
+        g = cls(h)
+        g._keep = record  # its closure holds the plans and arrays the graph points at
+        return g
 
     def launch(self, stream=None):
         check(lib().bcsr_graph_launch(self.handle, stream_handle(stream)), "bcsr_graph_launch")
