@@ -93,30 +93,34 @@ def batch_variant(precision_id, dev_bh, tile):
     """The kernel variant the batched persistent kernel runs for a device
     block height and tile width: the prefetching one (1) where its two load
     sets fit, else the plain one (0), as batch_prefetch in kernels_batch.hip.
-    FP8 has one kernel, timed as variant 1."""
+    FP8 has a single kernel, which its configurations name as variant -1
+    (CONFIGS_FP8)."""
+    if precision_id == 2:
+        return -1
     h = 2 if precision_id == 3 else 1
-    return 1 if precision_id == 2 or (dev_bh // 16) * (tile // 16) * h <= 16 else 0
+    return 1 if (dev_bh // 16) * (tile // 16) * h <= 16 else 0
+
+
+def best_batch_tile(configs, values, precision_id, dev_bh):
+    """The tile width of the smallest value (a time or a predicted log time)
+    among the configurations the persistent kernel runs for this device
+    block height: the variant batch_variant names, a tile width the batch is
+    compiled for, a finite value.  0 (the batch's automatic width) if none."""
+    best, best_v = 0, np.inf
+    for (tile, variant), v in zip(configs, values):
+        if v is None or not np.isfinite(v) or variant != batch_variant(precision_id, dev_bh, tile):
+            continue
+        if not lib().bcsr_batch_tile_supported(precision_id, dev_bh, tile):
+            continue
+        if v < best_v:
+            best, best_v = tile, v
+    return best
 
 
 def batch_tiles_from_times(plans, Ns, times, configs, precision_id):
     """Each problem's fastest tile width among those the persistent kernel
     supports, judged by the timings of the variant the batch runs there."""
-    tiles = []
-    for p, N, t in zip(plans, Ns, times):
-        best, best_ms = None, np.inf
-        dev_bh = p.info.dev_bh
-        for c, ms in zip(configs, t):
-            tile, variant = c
-            if not np.isfinite(ms) or variant != batch_variant(precision_id, dev_bh, tile):
-                continue
-            if not lib().bcsr_batch_tile_supported(precision_id, dev_bh, tile):
-                continue
-            if ms < best_ms:
-                best, best_ms = tile, ms
-        if best is None:
-            best = 0  # the batch's automatic width
-        tiles.append(best)
-    return tiles
+    return [best_batch_tile(configs, t, precision_id, p.info.dev_bh) for p, t in zip(plans, times)]
 
 
 def tune_batch(plans, Bs, Cs, tiles, stream, waves=BATCH_WAVES, bins=BATCH_BINS, graph=True,
@@ -160,5 +164,5 @@ def precision_id(precision):
 
 
 __all__ = ["CONFIGS_16", "CONFIGS_FP8", "TimingDB", "time_configs", "time_auto", "tune_batch",
-           "batch_variant", "batch_tiles_from_times", "tune_batch_policies", "configs_for", "cfg_of",
-           "supported_mask", "record", "check"]
+           "batch_variant", "best_batch_tile", "batch_tiles_from_times", "tune_batch_policies",
+           "configs_for", "cfg_of", "supported_mask", "record", "check"]

@@ -165,6 +165,11 @@ __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot
         if (g >= total) return false;
         tile = static_cast<int>(g / a.dev_rows);
         slot = static_cast<int>(g % a.dev_rows);
+    } else if (a.map_mode == 4) {
+        const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
+        const int tpg = a.n_tiles >> 1;
+        tile = (x & 1) * tpg + i % tpg;
+        slot = (i / tpg) * 4 + (x >> 1);
     } else {
         const int x = bid & 7, i = (bid >> 3) * a.wpb + w;
         if (a.n_tiles >= 8) {

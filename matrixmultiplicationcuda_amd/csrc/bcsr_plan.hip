@@ -491,7 +491,8 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
             map = xcd_ok ? 1 : 3;
     }
     if (map == 1 && !xcd_ok) return BCSR_ERR_INVALID_ARG;
-    if (map < 0 || map > 3) return BCSR_ERR_INVALID_ARG;
+    if (map == 4 && ls.n_tiles % 2) return BCSR_ERR_INVALID_ARG;
+    if (map < 0 || map > 4) return BCSR_ERR_INVALID_ARG;
     ls.map_mode = map;
     const int wpb = cfg && cfg->waves_per_block > 0 ? cfg->waves_per_block : 1;
     if (wpb != 1 && wpb != 2 && wpb != 4) return BCSR_ERR_INVALID_ARG;
@@ -505,6 +506,9 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
     } else if (map == 3) {
         const int per_xcd = static_cast<int>((static_cast<int64_t>(ls.n_tiles) * rows + 7) / 8);
+        ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
+    } else if (map == 4) {
+        const int per_xcd = (rows + 3) / 4 * (ls.n_tiles / 2);
         ls.grid = 8 * ((per_xcd + wpb - 1) / wpb);
     } else if (ls.n_tiles >= 8) {
         const int per_xcd = ls.n_tiles / 8 * rows;  // items of one XCD

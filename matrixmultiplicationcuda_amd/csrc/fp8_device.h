@@ -176,7 +176,6 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
     const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(a.B8, a.b_bytes);
     const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
     const unsigned lane_off = static_cast<unsigned>(n_base + j) * 16u;
-    const int sb = a.b8scale[n_base / kChunk];
 
     f32x4 acc[RS][EPL];
 #pragma unroll
@@ -186,6 +185,9 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 
     const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
     if (q0 < q1) {
+        // B's chunk scale exists only where B has rows (a row with octets
+        // implies K > 0; with K = 0 no quantised B is made)
+        const int sb = a.b8scale[n_base / kChunk];
         // slab offsets of this lane's two halves (slots g and 4 + g) in octet q;
         // the eight columns come in as two scalar 16-byte loads (clamped to the
         // last octet) and are picked per lane group by selects, not branches

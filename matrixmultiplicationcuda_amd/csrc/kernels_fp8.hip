@@ -131,8 +131,12 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     uint8_t *B8 = static_cast<uint8_t *>(plan->d_ws);
     uint8_t *bscale = B8 + off_s;
     float *part = reinterpret_cast<float *>(B8 + off_p);
-    int rc = enqueue_quant(B, K, N, B8, bscale, part, stream);
-    if (rc != BCSR_OK) return rc;
+    // K = 0: B has no rows to quantise; the product kernel writes C = 0
+    // without reading B or its scales
+    if (slabs > 0) {
+        int rc = enqueue_quant(B, K, N, B8, bscale, part, stream);
+        if (rc != BCSR_OK) return rc;
+    }
     return launch_fp8_product(plan, ls, B8, bscale, C, N, stream);
 }
 

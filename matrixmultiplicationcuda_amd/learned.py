@@ -257,19 +257,8 @@ class Dispatcher:
         from . import PRECISIONS, autotune
         prec = PRECISIONS[self.precision]
         pred = self.predict(list(zip(hosts, Ns)))
-        tiles = []
-        for p, N, row in zip(plans, Ns, pred):
-            dev_bh = p.info.dev_bh
-            best, best_v = 0, np.inf
-            for c, v in zip(self.configs, row):
-                tile, variant = c
-                if variant != autotune.batch_variant(prec, dev_bh, tile) or \
-                        not lib().bcsr_batch_tile_supported(prec, dev_bh, tile):
-                    continue
-                if v < best_v:
-                    best, best_v = tile, v
-            tiles.append(best)
-        return tiles
+        return [autotune.best_batch_tile(self.configs, row, prec, p.info.dev_bh)
+                for p, row in zip(plans, pred)]
 
     def close(self):
         self.model.close()

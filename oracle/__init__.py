@@ -67,12 +67,15 @@ def lib():
         f("orc_quant_e4m3_groups", None, _P, ctypes.c_int64, ctypes.c_int64, _P, _P)
         f("orc_quant_e4m3_B", None, _P, i, i, _P, _P)
         f("orc_quant_e4m3_tiles", None, _P, _P, _P, i, i, i, _P)
+        f("orc_quant_e4m3_blocks_f32scale", None, _P, ctypes.c_int64, ctypes.c_int64, _P)
         f("orc_csr_spmm_f32", None, _P, _P, _P, i, _P, i, _P)
         f("orc_csr_spmm_acc_f32", None, _P, _P, _P, i, _P, i, _P)
         f("orc_f2bf16", ctypes.c_uint16, ctypes.c_float)
         f("orc_bf162f", ctypes.c_float, ctypes.c_uint16)
         f("orc_hilo_split", None, ctypes.c_float, ctypes.POINTER(ctypes.c_uint16),
           ctypes.POINTER(ctypes.c_uint16))
+        f("orc_f2bf16_n", None, _P, ctypes.c_int64, _P)
+        f("orc_hilo_split_n", None, _P, ctypes.c_int64, _P, _P)
         _o = o
     return _o
 
@@ -316,14 +319,26 @@ def fp8_tiles_dequant(hdr, idx, data32, bh, bw):
     return out
 
 
+def fp8_blocks_f32scale_dequant(data32, blk):
+    """A's blocks quantised with one FP32 absmax scale each (the per-block
+    FP32 form of FP8 storage), dequantised; blk = values per block."""
+    data = np.ascontiguousarray(data32, dtype=np.float32)
+    out = np.zeros_like(data)
+    if data.size:
+        lib().orc_quant_e4m3_blocks_f32scale(_p(data), data.size // blk, blk, _p(out))
+    return out
+
+
 def e8m0_to_f32(e):
     return np.ldexp(np.float32(1.0), np.asarray(e, dtype=np.int32) - 127).astype(np.float32)
 
 
 def f32_to_bf16_bits(x):
     x = np.ascontiguousarray(x, dtype=np.float32)
-    return np.array([lib().orc_f2bf16(float(v)) for v in x.ravel()],
-                    dtype=np.uint16).reshape(x.shape)
+    out = np.empty(x.shape, dtype=np.uint16)
+    if x.size:
+        lib().orc_f2bf16_n(_p(x), x.size, _p(out))
+    return out
 
 
 def bf16_bits_to_f32(b):
@@ -334,10 +349,8 @@ def hilo_split(x):
     x = np.ascontiguousarray(x, dtype=np.float32)
     hi = np.empty(x.shape, np.uint16)
     lo = np.empty(x.shape, np.uint16)
-    h, l_ = ctypes.c_uint16(), ctypes.c_uint16()
-    for k, v in enumerate(x.ravel()):
-        lib().orc_hilo_split(float(v), ctypes.byref(h), ctypes.byref(l_))
-        hi.flat[k], lo.flat[k] = h.value, l_.value
+    if x.size:
+        lib().orc_hilo_split_n(_p(x), x.size, _p(hi), _p(lo))
     return hi, lo
 
 

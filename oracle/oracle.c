@@ -664,6 +664,23 @@ void orc_quant_e4m3_tiles(const int32_t *hdr, const int32_t *idx, const float *d
     }
 }
 
+/* One FP32 scale per block (s = absmax / 448, rounded to fp32), values
+ * quantised as E4M3(x / s) and dequantised as E4M3 * s in fp32. */
+void orc_quant_e4m3_blocks_f32scale(const float *data, int64_t nblk, int64_t blk, float *out) {
+    for (int64_t b = 0; b < nblk; ++b) {
+        const float *x = data + b * blk;
+        float *o = out + b * blk;
+        float amax = 0.0f;
+        for (int64_t i = 0; i < blk; ++i) amax = fmaxf(amax, fabsf(x[i]));
+        if (!(amax > 0.0f)) {
+            for (int64_t i = 0; i < blk; ++i) o[i] = 0.0f;
+            continue;
+        }
+        const float s = amax / 448.0f;
+        for (int64_t i = 0; i < blk; ++i) o[i] = orc_e4m32f(orc_f2e4m3(x[i] / s)) * s;
+    }
+}
+
 /* bfloat16 and hi+lo fp16 storage (extensions). */
 uint16_t orc_f2bf16(float f) {
     uint32_t x;
@@ -683,4 +700,13 @@ float orc_bf162f(uint16_t b) {
 void orc_hilo_split(float a, uint16_t *hi, uint16_t *lo) {
     *hi = orc_f2h(a);
     *lo = orc_f2h((a - orc_h2f(*hi)) * 2048.0f);
+}
+
+/* The same conversions over arrays. */
+void orc_f2bf16_n(const float *x, int64_t n, uint16_t *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_f2bf16(x[i]);
+}
+
+void orc_hilo_split_n(const float *x, int64_t n, uint16_t *hi, uint16_t *lo) {
+    for (int64_t i = 0; i < n; ++i) orc_hilo_split(x[i], hi + i, lo + i);
 }

@@ -172,6 +172,15 @@ void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale);
 void orc_quant_e4m3_tiles(const int32_t *hdr, const int32_t *idx, const float *data, int block_rows, int bh,
                           int bw, float *out);
 
+/* The per-block FP32-scale form of FP8 storage (measured against the E8M0
+ * tile form above; the kernels do not use it): per block of `blk` floats,
+ * s = absmax / 448 in fp32, out = E4M3(x / s) * s (all zeros for a zero
+ * block).  pinned (tests/test_oracle_precision.py): equals
+ * orc_quant_e4m3_groups' result wherever s is a power of two, the block's
+ * largest element maps to +-448 * s, and every value stays within E4M3's
+ * rounding bound (2^-4 relative above 2^-6 s, 2^-10 s below). */
+void orc_quant_e4m3_blocks_f32scale(const float *data, int64_t nblk, int64_t blk, float *out);
+
 /* ---- other storage modes (extensions) ---------------------------------
  * bf16: float -> bfloat16 round to nearest even.  hi+lo: hi = fp16(a),
  * lo = fp16((a - hi) * 2^11), a ~ hi + 2^-11 lo.  pinned
@@ -181,6 +190,9 @@ void orc_quant_e4m3_tiles(const int32_t *hdr, const int32_t *idx, const float *d
 uint16_t orc_f2bf16(float f);
 float orc_bf162f(uint16_t b);
 void orc_hilo_split(float a, uint16_t *hi, uint16_t *lo);
+/* orc_f2bf16 / orc_hilo_split applied to n values. */
+void orc_f2bf16_n(const float *x, int64_t n, uint16_t *out);
+void orc_hilo_split_n(const float *x, int64_t n, uint16_t *hi, uint16_t *lo);
 
 #ifdef __cplusplus
 }

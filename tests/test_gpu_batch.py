@@ -171,3 +171,39 @@ def test_batch_errors():
     empty.close()
     p8.close()
     teardown(plans, Bs, Cs)
+
+
+def test_fp8_with_empty_inner_dimension():
+    """FP8 products with K = 0 (no block-columns): C = 0, per matrix and in a
+    batch beside an ordinary problem; no quantiser runs and no B scale is
+    read."""
+    mm.check(mm.lib().bcsr_set_device(0))
+    N, rows = 64, 64
+    empty = mm.HostBCSR(np.zeros(rows // 16 + 1, np.int32), np.zeros(0, np.int32),
+                        np.zeros((0, 16, 16), np.float32), rows // 16, 0, 16, 16)
+    log("fp8 K = 0, per matrix")
+    p0 = mm.Plan(empty, "fp8")
+    Bdummy = mm.DeviceArray((1, N), np.float16)
+    C0 = mm.DeviceArray((rows, N), np.float32)
+    C0.fill_bytes(0xFF)
+    p0.spmm_ptr(Bdummy.ptr, C0.ptr, N)
+    mm.synchronize()
+    assert np.all(C0.numpy() == 0.0)
+    log("fp8 K = 0 in a batch")
+    other = small_problems(1, seed=11)[0]
+    h1 = other.bcsr(np.float32)
+    p1 = mm.Plan(h1, "fp8")
+    B1 = mm.DeviceArray.from_numpy(other.B())
+    C1 = mm.DeviceArray((other.rows, other.N), np.float32)
+    C0.fill_bytes(0xFF)
+    batch = mm.Batch([p0, p1], [Bdummy, B1], [C0, C1])
+    batch.run()
+    mm.synchronize()
+    assert np.all(C0.numpy() == 0.0)
+    want = p1.matmul(B1.numpy(), cfg=mm.make_cfg(tile_n=int(batch.tiles[1]), variant=-1))
+    assert np.array_equal(C1.numpy(), want)
+    batch.close()
+    for x in (p0, p1):
+        x.close()
+    for a in (Bdummy, C0, B1, C1):
+        a.free()

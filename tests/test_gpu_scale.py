@@ -1,0 +1,247 @@
+"""GPU parity at the sizes the at-scale workloads are quoted on.
+
+1. The plain fp16 product at 16384 x 16384 over every cell of the at-scale
+   table: block shapes 16 x 16, 32 x 16, 64 x 16; random blocks at 1, 5 and
+   20 % ("blockrandom" / "pattern32by16" / "pattern64by16", seed 123, the
+   generator's block size = bh as prepare_buffers does, nvbench_main.cu:
+   139-175), the half-width-8 block band and the 256 x 256 block-diagonal;
+   N = 64, 256, 512 (B seed 456).  Each cell: 40 sampled block-rows (first,
+   last and the longest among them) against the oracle's CPU product at
+   1e-3 of (|A||B|)_ij with exact zeros (the reference's own check is 1e-2
+   absolute, nvbench_main.cu:411-420), and the row-sum identity
+   C.1 = A.(B.1) over every row.
+2. The precision workload at full size: 16384^2, 16 x 16 blocks, 5 %, block
+   magnitudes log-uniform on [1e-3, 1e2] (seed 321), N = 512, in fp16, bf16,
+   hi+lo and FP8: sampled rows against the exact product of the values each
+   mode stores, and every entry against FP64 with each mode's bound (FP8:
+   max, mean and normwise error, bounds within 1.5x of the measured
+   1.93e-2 / 2.8e-3 / 3.6e-2).
+3. The batched workload in full: the 96 problems of workloads.draw_problems
+   (96, 1000) in one persistent launch, fp16 and FP8: every C equal bit for
+   bit to its own per-matrix launch with the batch's tile; fp16 problems on
+   sampled rows against the oracle; FP8 against FP64 on all 96.
+"""
+import concurrent.futures as cf
+import sys
+
+import numpy as np
+import pytest
+
+import matrixmultiplicationcuda_amd as mm
+import oracle
+from matrixmultiplicationcuda_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+M = K = 16384
+RANDOM_PATTERN = {16: "blockrandom", 32: "pattern32by16", 64: "pattern64by16"}
+CELLS = [("random", 0.01), ("random", 0.05), ("random", 0.20), ("band", None), ("blockdiag", None)]
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_matrix(bh, kind, density):
+    if kind == "random":
+        h = mm.generate_bcsr(M, K, 1.0 - density, RANDOM_PATTERN[bh], bh=bh, bw=16, seed=123)
+    elif kind == "band":
+        h = mm.generate_bcsr(M, K, 0.0, "blockband", bh=bh, bw=16, seed=123, band_halfwidth=8)
+    else:
+        A = mm.generate(M, K, 0.0, "blockdiagonal", bh=256, seed=123)
+        return mm.bcsr_from_dense(A, bh, 16)
+    return mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, bh, 16)
+
+
+_cache = {}
+
+
+def cached(key, make):
+    """One matrix (and its plan) and one B at a time: the cells of a matrix
+    run one after another, so the previous matrix is released first."""
+    if key not in _cache:
+        for k in [k for k in _cache if k[0] == key[0]]:
+            v = _cache.pop(k)
+            if isinstance(v, tuple) and hasattr(v[1], "close"):
+                v[1].close()
+        _cache[key] = make()
+    return _cache[key]
+
+
+def row_sums(h, B):
+    """(sum_j C[i, j], sum_j (|A||B|)[i, j]) in fp64 from A and the row sums
+    of B: C.1 = A.(B.1)."""
+    bsum = B.astype(np.float64).sum(axis=1).reshape(-1, h.bw)
+    babs = np.abs(B.astype(np.float64)).sum(axis=1).reshape(-1, h.bw)
+    data = h.data.astype(np.float64)
+    blk_rows = np.repeat(np.arange(h.block_rows), np.diff(h.hdr))
+    per_blk = np.einsum("kij,kj->ki", data, bsum[h.idx])
+    per_abs = np.einsum("kij,kj->ki", np.abs(data), babs[h.idx])
+    out = np.zeros((h.block_rows, h.bh))
+    out_abs = np.zeros((h.block_rows, h.bh))
+    np.add.at(out, blk_rows, per_blk)
+    np.add.at(out_abs, blk_rows, per_abs)
+    return out.ravel(), out_abs.ravel()
+
+
+def check_rows(C, h, B16, brs, tol=1e-3, what=""):
+    Bf = B16.astype(np.float32)
+    data16 = h.data.view(np.uint16)
+    data32 = h.data.astype(np.float32)
+    worst = 0.0
+    for br in brs:
+        ref = oracle.bcsr_spmm_f32(h.hdr, h.idx, data16, h.bh, h.bw, Bf, br, br + 1, threads=8)
+        _, cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, data32, h.bh, h.bw, Bf, br, br + 1, threads=8)
+        sl = slice(br * h.bh, (br + 1) * h.bh)
+        got = C[sl].astype(np.float64)
+        err = np.abs(got - ref[sl])
+        zero = cabs[sl] == 0
+        assert np.all(got[zero] == 0), f"{what}: block-row {br}: nonzero where |A||B| = 0"
+        assert np.all(err <= tol * cabs[sl]), f"{what}: block-row {br}: max err/|A||B| " \
+            f"{np.max(err / np.maximum(cabs[sl], 1e-30)):.3e}"
+        assert err.max(initial=0.0) <= 1e-2, f"{what}: the reference's 1e-2 check"
+        if (~zero).any():
+            worst = max(worst, float((err[~zero] / cabs[sl][~zero]).max()))
+    return worst
+
+
+@pytest.mark.parametrize("N", [64, 256, 512])
+@pytest.mark.parametrize("kind,density", CELLS)
+@pytest.mark.parametrize("bh", [16, 32, 64])
+def test_at_scale_cell(bh, kind, density, N):
+    assert mm.device_count() > 0
+    mm.check(mm.lib().bcsr_set_device(0))
+    log(f"cell bh={bh} {kind} {density} N={N}")
+    h, plan = cached(("A", bh, kind, density), lambda: (lambda h: (h, mm.Plan(h, "f16")))(
+        make_matrix(bh, kind, density)))
+    B16 = cached(("B", N), lambda: mm.generate(K, N, 0.0, "random", bh=16, seed=456))
+    log(f"  nnzb={h.nnzb} kernel={plan.kernel_name(N)}")
+    dB = mm.DeviceArray.from_numpy(B16)
+    dC = mm.DeviceArray((M, N), np.float32)
+    dC.fill_bytes(0xFF)  # NaN: every element must be written
+    plan.spmm_ptr(dB.ptr, dC.ptr, N)
+    mm.synchronize()
+    C = dC.numpy()
+    dB.free()
+    dC.free()
+    assert np.isfinite(C).all()
+    cnt = np.diff(h.hdr)
+    rng = np.random.default_rng(bh * 1000 + N)
+    brs = sorted(set(rng.choice(h.block_rows, 37, replace=False).tolist())
+                 | {0, h.block_rows - 1, int(np.argmax(cnt))})
+    worst = check_rows(C, h, B16, brs, what=f"{bh}x16 {kind} {density} N={N}")
+    ref, ref_abs = row_sums(h, B16)
+    got = C.astype(np.float64).sum(axis=1)
+    assert np.all(np.abs(got - ref) <= 1e-3 * ref_abs + 1e-6), "row sums"
+    log(f"  sampled block-rows {len(brs)}: max err/|A||B| {worst:.2e}; row sums ok")
+
+
+# ---- the precision workload at full size ----------------------------------------
+BOUND_VS_FP64 = {"f16": 2.0**-11 + 1e-6, "bf16": 2.0**-8 * 2 + 2.0**-16 + 1e-6, "hilo": 5e-5}
+FP8_BOUNDS = {"max": 0.029, "mean": 4.2e-3, "normwise": 0.054}
+
+
+@pytest.fixture(scope="module")
+def precision_workload():
+    h = mm.generate_bcsr(M, K, 0.95, "blockrandom", bh=16, bw=16, seed=321, value_dist="loguniform",
+                         lo=1e-3, hi=1e2)
+    B16 = mm.generate(K, 512, 0.0, "random", bh=16, seed=456)
+    Bf = B16.astype(np.float32)
+    C64, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data, 16, 16, Bf, threads=16)
+    return h, B16, C64, Cabs
+
+
+@pytest.mark.parametrize("mode", ["f16", "bf16", "hilo", "fp8"])
+def test_precision_modes_at_full_size(mode, precision_workload):
+    h, B16, C64, Cabs = precision_workload
+    log(f"precision workload {mode}: nnzb={h.nnzb}")
+    mm.check(mm.lib().bcsr_set_device(0))
+    plan = mm.Plan(h, mode)
+    C = plan.matmul(B16).astype(np.float64)
+    plan.close()
+    # against the exact product of the stored values, on sampled block-rows
+    # (each row's blocks quantised as the mode stores them: FP8 scales pair
+    # consecutive blocks of a block-row)
+    Bq = oracle.quantized_B(mode, B16.view(np.uint16))
+    brs = np.random.default_rng(3).choice(h.block_rows, 24, replace=False)
+    tight = 2.0**-11 if mode == "fp8" else 1e-4
+    for br in brs:
+        k0, k1 = int(h.hdr[br]), int(h.hdr[br + 1])
+        hdr1 = np.array([0, k1 - k0], np.int32)
+        idx1 = np.ascontiguousarray(h.idx[k0:k1])
+        Aq = np.asarray(oracle.dequantized_blocks(mode, h.data[k0:k1], 16, 16, hdr1, idx1), np.float32)
+        Cq, Cqa = oracle.bcsr_spmm_f64(hdr1, idx1, Aq, 16, 16, Bq, threads=16)
+        sl = slice(16 * br, 16 * br + 16)
+        assert np.all(np.abs(C[sl] - Cq) <= tight * Cqa + 1e-30), f"{mode}: block-row {br}"
+    nz = Cabs > 0
+    rel = np.abs(C - C64)[nz] / Cabs[nz]
+    e = {"max": float(rel.max()), "mean": float(rel.mean()),
+         "normwise": float(np.linalg.norm(C - C64) / np.linalg.norm(C64))}
+    log(f"  {mode} vs FP64: max {e['max']:.3e} mean {e['mean']:.3e} normwise {e['normwise']:.3e}")
+    if mode == "fp8":
+        for k, b in FP8_BOUNDS.items():
+            assert e[k] <= b, (k, e)
+    else:
+        assert e["max"] <= BOUND_VS_FP64[mode], e
+
+
+# ---- the batched workload in full ----------------------------------------------
+@pytest.fixture(scope="module")
+def batch_workload():
+    probs = workloads.draw_problems(96, base_seed=1000)
+    with cf.ThreadPoolExecutor(max_workers=12) as ex:
+        built = list(ex.map(lambda p: (p.bcsr(np.float32), p.B()), probs))
+    return probs, built
+
+
+@pytest.mark.parametrize("precision", ["f16", "fp8"])
+def test_full_batch(precision, batch_workload):
+    probs, built = batch_workload
+    mm.check(mm.lib().bcsr_set_device(0))
+    hosts = []
+    for h, _ in built:
+        if precision == "f16":
+            h = mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, h.bh, h.bw)
+        hosts.append(h)
+    plans = [mm.Plan(h, precision) for h in hosts]
+    Bs = [mm.DeviceArray.from_numpy(b) for _, b in built]
+    Cs = [mm.DeviceArray((p.rows, p.N), np.float32) for p in probs]
+    for C in Cs:
+        C.fill_bytes(0xFF)
+    batch = mm.Batch(plans, Bs, Cs)
+    log(f"batch {precision}: {len(probs)} problems, items {batch.info.items}, "
+        f"{batch.info.flops / 1e9:.1f} GFLOP")
+    batch.run(graph=True)
+    mm.synchronize()
+    tiles = batch.tiles
+    got = [C.numpy() for C in Cs]
+    for i, (pl, B, t) in enumerate(zip(plans, Bs, tiles)):
+        C1 = mm.DeviceArray((probs[i].rows, probs[i].N), np.float32)
+        pl.spmm_ptr(B.ptr, C1.ptr, probs[i].N, None, mm.make_cfg(tile_n=int(t), variant=-1))
+        mm.synchronize()
+        assert np.array_equal(got[i], C1.numpy()), f"problem {i} ({probs[i]}) differs from its own launch"
+        C1.free()
+    if precision == "f16":
+        worst = 0.0
+        for i, h in enumerate(hosts):
+            brs = np.random.default_rng(i).choice(h.block_rows, min(4, h.block_rows), replace=False)
+            worst = max(worst, check_rows(got[i], h, built[i][1], sorted(brs.tolist()), what=f"problem {i}"))
+        log(f"  fp16 sampled rows vs the oracle: max err/|A||B| {worst:.2e}")
+    else:
+        emax, means = 0.0, []
+        for i, (h, B16) in enumerate(built):
+            C64, Cabs = oracle.bcsr_spmm_f64(h.hdr, h.idx, h.data, h.bh, h.bw, B16.astype(np.float32),
+                                             threads=16)
+            nz = Cabs > 0
+            rel = np.abs(got[i] - C64)[nz] / Cabs[nz]
+            if rel.size:
+                emax = max(emax, float(rel.max()))
+                means.append(float(rel.mean()))
+        log(f"  FP8 vs FP64 over 96 problems: max {emax:.3e}, mean of means {np.mean(means):.3e}")
+        assert np.mean(means) <= 2e-2 and emax <= 0.25
+    batch.close()
+    for x in plans:
+        x.close()
+    for a in Bs + Cs:
+        a.free()

@@ -308,3 +308,50 @@ def test_dispatcher_choice_edge_cases():
     assert fb
     k, fb = learned.choose(np.array([1.0, 2.0, 0.0, 3.0]), np.array([0, 1, 0, 0], np.float32))
     assert k == 1 and not fb  # a single supported configuration
+
+
+class _StubInfo:
+    def __init__(self, dev_bh):
+        self.dev_bh = dev_bh
+
+
+class _StubPlan:
+    def __init__(self, dev_bh):
+        self.info = _StubInfo(dev_bh)
+
+
+@pytest.mark.parametrize("precision", ["f16", "fp8"])
+def test_batch_tile_pickers_take_the_fastest_batch_tile(precision):
+    """Both batch tile pickers (timings, cost-model predictions) pick the
+    fastest configuration among those the persistent kernel runs: for FP8
+    (whose configurations carry variant -1) as well as for fp16."""
+    from matrixmultiplicationcuda_amd import autotune
+
+    prec = mm.PRECISIONS[precision]
+    configs = autotune.configs_for(precision)
+    for dev_bh in (16, 32, 64):
+        batch_tiles = [t for t in autotune.TILES
+                       if mm.lib().bcsr_batch_tile_supported(prec, dev_bh, t)]
+        assert batch_tiles
+        for want in batch_tiles:
+            # every configuration 1 ms except the batch's variant of `want`;
+            # other variants of the same tile are faster still but not run
+            # by the batch, and 256 columns (never batched) fastest of all
+            t = np.ones(len(configs))
+            for k, (tile, variant) in enumerate(configs):
+                if tile == want:
+                    t[k] = 0.5 if variant == autotune.batch_variant(prec, dev_bh, tile) else 0.1
+                if tile == 256:
+                    t[k] = 0.01
+            got = autotune.batch_tiles_from_times([_StubPlan(dev_bh)], [512], [t], configs, prec)
+            assert got == [want], (precision, dev_bh, want)
+            assert autotune.best_batch_tile(configs, np.log(t), prec, dev_bh) == want
+        # nothing timed: the batch's automatic width
+        assert autotune.best_batch_tile(configs, [np.nan] * len(configs), prec, dev_bh) == 0
+
+
+def test_quantize_b_with_no_rows_is_a_no_op():
+    """bcsr_quantize_b_e4m3 with K = 0 (or N = 0) returns before any HIP call."""
+    assert mm.lib().bcsr_quantize_b_e4m3(None, 0, 64, None, None, None, None) == 0
+    assert mm.lib().bcsr_quantize_b_e4m3(None, 64, 0, None, None, None, None) == 0
+    assert mm.lib().bcsr_quantize_b_e4m3(None, 8, 64, None, None, None, None) == -3

@@ -288,3 +288,43 @@ def test_planted_mistakes_are_caught():
     hdr2[0] = 0
     hdr2 = np.maximum(hdr2, 0)
     assert caught(oracle.bcsr_spmm_f32(hdr2, idx[1:], data[1:], 16, 16, Bf))
+
+
+# ---- the 16384^2 workloads the bench and the GPU tests time -----------------------
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["c1_16x16_d1", "c1_16x16_d5", "c1_16x16_d20", "c1_32x16_d5",
+                                  "c1_64x16_d5", "c1_blockdiag256"])
+def test_library_generator_matches_large_golden(name):
+    """The library's generator (mm.generate_bcsr, or the dense generator plus
+    BCSRMatrix for the 256 x 256 block-diagonal) gives, at 16384^2, the
+    arrays whose hashes were written from the reference's own
+    matrix_generator.cpp (matrix_generator.cpp:28-140, formats/BCSRMatrix.cu:22-74)."""
+    import matrixmultiplicationcuda_amd as mm
+
+    g = golden()["large"][name]
+    if g["pattern"] == "blockdiagonal":
+        A = mm.generate(g["M"], g["K"], g["sparsity"], g["pattern"], bh=g["blocksize"],
+                        seed=g["seed"])
+        h = mm.bcsr_from_dense(A, g["bh"], g["bw"])
+        del A
+        data16 = np.ascontiguousarray(h.data, dtype=np.float16)
+    else:
+        assert g["blocksize"] == g["bh"]
+        h = mm.generate_bcsr(g["M"], g["K"], g["sparsity"], g["pattern"], bh=g["bh"], bw=g["bw"],
+                             seed=g["seed"])
+        data16 = np.ascontiguousarray(h.data.astype(np.float16))
+    assert h.nnzb == g["nnzb"]
+    assert sha(np.ascontiguousarray(h.hdr, np.int32)) == g["sha_hdr"]
+    assert sha(np.ascontiguousarray(h.idx, np.int32)) == g["sha_idx"]
+    assert sha(data16.view(np.uint16)) == g["sha_data"]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["B_1024x256", "B_16384x64", "B_16384x256", "B_16384x512"])
+def test_library_B_matches_golden(name):
+    """mm.generate's B (seed 456, fp16) at every N the bench and tests use."""
+    import matrixmultiplicationcuda_amd as mm
+
+    g = golden()["B"][name]
+    B = mm.generate(g["K"], g["N"], 0.0, "random", bh=16, seed=g["seed"])
+    assert sha(np.ascontiguousarray(B).view(np.uint16)) == g["sha_f16"]
