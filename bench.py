@@ -55,7 +55,14 @@ def parse():
     p.add_argument("--no-extensions", action="store_true",
                    help="skip the precision-mode and batched/learned measurements")
     p.add_argument("--batch-count", type=int, default=96)
-    p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="0 = every CPU this process may run on (its affinity set)")
+    p.add_argument("--timing", default="cold", choices=["cold", "warm"],
+                   help="the headline: cold = before every timed launch, outside its events, a "
+                        "copy over 2 x 512 MiB evicts A, B and C from L2 and the 256 MB Infinity "
+                        "Cache (the reference's NVBench cold sampling); warm = back-to-back "
+                        "launches.  Both are measured and reported; this picks `value`.")
+    p.add_argument("--flush-mb", type=int, default=512)
     return p.parse_args()
 
 
@@ -67,6 +74,56 @@ def algorithmic_bytes(nnzb, bh, bw, M, K, N, precision="f16"):
     scales = 4 * nnzb if precision == "fp8" else 0
     return (nnzb * bh * bw * A_BYTES[precision] + scales + 4 * nnzb + 4 * (M // bh + 1)
             + K * N * 2 + M * N * 4)
+
+
+def timed_launches(mm, np, fn, stream, steps, flush=None):
+    """Per-launch milliseconds of `steps` launches of fn (HIP events around
+    each launch on its stream).  flush(), when given, runs on the same stream
+    before each launch, outside that launch's events.  Returns the list of
+    times and the wall-clock seconds of the whole loop (between two
+    synchronisations)."""
+    ev = [(mm.Event(), mm.Event()) for _ in range(steps)]
+    stream.synchronize()
+    mm.synchronize()
+    w0 = time.perf_counter()
+    for e0, e1 in ev:
+        if flush is not None:
+            flush()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+    stream.synchronize()
+    mm.synchronize()
+    wall = time.perf_counter() - w0
+    ms = [e0.elapsed_ms(e1) for e0, e1 in ev]
+    for e0, e1 in ev:
+        e0.destroy()
+        e1.destroy()
+    return ms, wall
+
+
+def host_cpu():
+    """CPU model, logical CPUs, the CPUs this process may run on and the
+    cgroup CPU quota (None if unlimited or unreadable)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                quota = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "cgroup_cpu_quota": quota}
 
 
 def time_ms(stream, fn, warmup=2, reps=10):
@@ -83,35 +140,80 @@ def time_ms(stream, fn, warmup=2, reps=10):
     return e0.elapsed_ms(e1) / reps
 
 
-def precision_modes(mm, np, h16, h32, B, N, stream, ref_ms):
-    """The target matrix in bf16, FP8 and hi+lo storage (automatic launch)."""
-    out = {}
-    flops = 2.0 * h16.nnzb * h16.bh * h16.bw * N
-    M = h16.block_rows * h16.bh
-    for prec in ("bf16", "fp8", "hilo"):
+def fp64_reference(np, h, B16):
+    """(C, |A||B|) in float64 by scipy's BSR product (an independent check,
+    not the oracle)."""
+    import scipy.sparse as sps
+    M, K = h.block_rows * h.bh, h.block_cols * h.bw
+    d = h.data.astype(np.float64)
+    Bd = B16.astype(np.float64)
+    A = sps.bsr_matrix((d, h.idx, h.hdr), shape=(M, K))
+    Aa = sps.bsr_matrix((np.abs(d), h.idx, h.hdr), shape=(M, K))
+    return np.asarray(A @ Bd), np.asarray(Aa @ np.abs(Bd))
+
+
+def error_vs(np, C, C64, Cabs):
+    nz = Cabs > 0
+    rel = np.abs(C.astype(np.float64) - C64)[nz] / Cabs[nz]
+    return {"max": float(rel.max(initial=0.0)), "mean": float(rel.mean()) if rel.size else 0.0,
+            "normwise": float(np.linalg.norm(C - C64) / max(np.linalg.norm(C64), 1e-300))}
+
+
+def precision_modes(mm, np, N, stream, ref, flush, steps=50):
+    """The precision workload: 16384^2, 16 x 16 blocks, 5 % ("blockrandom",
+    seed 321), block magnitudes log-uniform on [1e-3, 1e2] with random sign,
+    B = U(-1, 1) fp16 (seed 456), N = 512; fp16, bf16, hi+lo and FP8 storage.
+    Each mode: cold and warm median launch times, TFLOPS, speed over the
+    fp16 headline, and the error of its output against the FP64 product of
+    the unquantised values (max and mean of |C - C64| / (|A||B|) per entry,
+    and normwise ||C - C64|| / ||C64||)."""
+    M = K = 16384
+    h32 = mm.generate_bcsr(M, K, 0.95, "blockrandom", bh=16, bw=16, seed=321, value_dist="loguniform",
+                           lo=1e-3, hi=1e2)
+    B16 = mm.generate(K, N, 0.0, "random", bh=16, seed=456)
+    C64, Cabs = fp64_reference(np, h32, B16)
+    B = mm.DeviceArray.from_numpy(B16)
+    C = mm.DeviceArray((M, N), np.float32)
+    flops = 2.0 * h32.nnzb * 256 * N
+    out = {"workload": "16384^2, 16 x 16 blocks, 5 % blockrandom (seed 321), |a| log-uniform on "
+                       "[1e-3, 1e2] with random sign, B U(-1, 1) fp16 (seed 456), N = " + str(N),
+           "nnzb": int(h32.nnzb), "fp64_reference": "scipy.sparse BSR product in float64"}
+    for prec in ("f16", "bf16", "fp8", "hilo"):
         plan = mm.Plan(h32, prec)
-        C = mm.DeviceArray((M, N), np.float32)
-        ms = time_ms(stream, lambda: plan.spmm_ptr(B.ptr, C.ptr, N, stream), reps=20)
-        out[prec] = {"ms": round(ms, 5), "tflops": round(flops / ms / 1e9, 2),
-                     "vs_f16": round(ref_ms / ms, 3), "kernel": plan.kernel_name(N)}
+        fn = lambda: plan.spmm_ptr(B.ptr, C.ptr, N, stream)
+        fn()
+        cold, _ = timed_launches(mm, np, fn, stream, steps, flush=flush)
+        warm, _ = timed_launches(mm, np, fn, stream, steps)
+        got = C.numpy()
+        cm, wm = float(np.median(cold)), float(np.median(warm))
+        out[prec] = {"cold_ms": round(cm, 5), "warm_ms": round(wm, 5),
+                     "tflops_cold": round(flops / cm / 1e9, 2), "tflops_warm": round(flops / wm / 1e9, 2),
+                     "vs_f16_headline_cold": round(ref["cold"] / cm, 3),
+                     "vs_f16_headline_warm": round(ref["warm"] / wm, 3),
+                     "kernel": plan.kernel_name(N), "error_vs_fp64": error_vs(np, got, C64, Cabs)}
         if prec == "fp8":
             # B quantised once (bcsr_quantize_b_e4m3), then only the product
             # (bcsr_spmm_b8): the cost per product when B is reused
-            K = h16.block_cols * h16.bw
             chunks = (N + 255) // 256
             d8 = mm.DeviceArray((K * N,), np.uint8)
             ds = mm.DeviceArray((chunks,), np.uint8)
             work = mm.DeviceArray((K // 16 * chunks,), np.float32)
             mm.check(mm.lib().bcsr_quantize_b_e4m3(B.ptr, K, N, d8.ptr, ds.ptr, work.ptr, stream.handle))
-            fn = lambda: mm.check(mm.lib().bcsr_spmm_b8(plan._handle, d8.ptr, ds.ptr, C.ptr, N, None,
-                                                        stream.handle))
-            ms8 = time_ms(stream, fn, reps=20)
-            out["fp8_prequantised_b"] = {"ms": round(ms8, 5), "tflops": round(flops / ms8 / 1e9, 2),
-                                         "vs_f16": round(ref_ms / ms8, 3), "kernel": plan.kernel_name(N)}
+            fn8 = lambda: mm.check(mm.lib().bcsr_spmm_b8(plan._handle, d8.ptr, ds.ptr, C.ptr, N, None,
+                                                         stream.handle))
+            fn8()
+            cold8, _ = timed_launches(mm, np, fn8, stream, steps, flush=flush)
+            warm8, _ = timed_launches(mm, np, fn8, stream, steps)
+            c8, w8 = float(np.median(cold8)), float(np.median(warm8))
+            out["fp8_prequantised_b"] = {"cold_ms": round(c8, 5), "warm_ms": round(w8, 5),
+                                         "tflops_cold": round(flops / c8 / 1e9, 2),
+                                         "vs_f16_headline_cold": round(ref["cold"] / c8, 3),
+                                         "vs_f16_headline_warm": round(ref["warm"] / w8, 3)}
             for a in (d8, ds, work):
                 a.free()
-        C.free()
         plan.close()
+    B.free()
+    C.free()
     return out
 
 
@@ -222,7 +324,7 @@ def published_shapes(mm, np, stream):
             "min_speedup_vs_reference": min(sp_list), "rows": rows}
 
 
-def batched(mm, np, count, stream):
+def batched(mm, np, count, stream, headline_tflops):
     """The batched uneven workload: plain fp16 per-matrix launches (the
     library's static choice each) against the batched persistent kernel
     (graph replays), fp16 and FP8.  The batch's tile widths come from the
@@ -281,26 +383,16 @@ def batched(mm, np, count, stream):
                  "tile_policy": best["policy"], "waves_per_cu": best["waves_per_cu"],
                  "bin_work": best["bin_work"],
                  "policies_ms": {k: round(v["ms"], 4) for k, v in table.items()}}
-        if prec == "fp8":
-            # error of the batched FP8 results against the FP64 product (3 problems)
-            batch.run(stream)
-            stream.synchronize()
-            errs = []
-            for i in sorted(range(count), key=lambda i: probs[i].rows * probs[i].N)[:3]:
-                h = h32s[i]
-                Cg = Cs[i].numpy().astype(np.float64)
-                # FP64 product of the generator's values (dense, numpy)
-                A = np.zeros((h.block_rows, h.bh, h.block_cols, h.bw), np.float64)
-                A[np.repeat(np.arange(h.block_rows), np.diff(h.hdr)), :, h.idx, :] = h.data
-                A = A.reshape(h.block_rows * h.bh, h.block_cols * h.bw)
-                B64 = built[i][1].astype(np.float64)
-                C64, Ca = A @ B64, np.abs(A) @ np.abs(B64)
-                nz = Ca > 0
-                e = np.abs(Cg - C64)[nz] / Ca[nz]
-                errs.append((float(e.max()), float(e.mean())))
-            entry["error_vs_fp64_per_entry"] = {"max": max(e[0] for e in errs),
-                                                "mean": float(np.mean([e[1] for e in errs])),
-                                                "problems": 3}
+        # error of every batched result against the FP64 product of the
+        # generator's values (scipy's BSR product in float64)
+        batch.run(stream)
+        stream.synchronize()
+        errs = [error_vs(np, Cs[i].numpy(), *fp64_reference(np, h32s[i], built[i][1]))
+                for i in range(count)]
+        entry["error_vs_fp64"] = {"max": max(e["max"] for e in errs),
+                                  "mean_of_means": float(np.mean([e["mean"] for e in errs])),
+                                  "max_normwise": max(e["normwise"] for e in errs),
+                                  "problems": count}
         batch.close()
         for pl in plans:
             pl.close()
@@ -308,8 +400,21 @@ def batched(mm, np, count, stream):
     for a in Bs + Cs:
         a.free()
     res.update(out)
-    res["speedup_fp8_batched_vs_fp16_per_matrix"] = round(
-        out["f16"]["per_matrix_ms"] / out["fp8"]["batched_ms"], 3)
+    f16, f8 = out["f16"], out["fp8"]
+    res["speedup_fp8_batched_vs_fp16_per_matrix"] = round(f16["per_matrix_ms"] / f8["batched_ms"], 3)
+    # the north-star ratio, FP8 + batched + learned TFLOPS over the plain fp16
+    # kernel's headline TFLOPS, and the factors it is made of on this workload
+    res["north_star_ratio"] = {
+        "fp8_batched_tflops_over_fp16_headline_cold": round(f8["batched_tflops"] / headline_tflops["cold"], 3),
+        "fp8_batched_tflops_over_fp16_headline_warm": round(f8["batched_tflops"] / headline_tflops["warm"], 3),
+        "factors_on_this_workload": {
+            "batching_fp16": round(f16["per_matrix_ms"] / f16["batched_ms"], 3),
+            "fp8_over_fp16_both_batched": round(f16["batched_ms"] / f8["batched_ms"], 3),
+            "learned_tiles_over_automatic_fp16": round(f16["policies_ms"]["auto"] / f16["policies_ms"]["learned"], 3),
+            "learned_tiles_over_automatic_fp8": round(f8["policies_ms"]["auto"] / f8["policies_ms"]["learned"], 3),
+            "per_matrix_learned_over_static_fp16": round(f16["per_matrix_ms"] / f16["per_matrix_learned_ms"], 3),
+            "per_matrix_learned_over_static_fp8": round(f8["per_matrix_ms"] / f8["per_matrix_learned_ms"], 3)},
+        "policy_used": {"f16": f16["tile_policy"], "fp8": f8["tile_policy"]}}
     return res
 
 
@@ -351,15 +456,29 @@ def main():
     plan.spmm_repeat_ptr(B.ptr, C.ptr, N, args.warmup, stream, cfg)
     stream.synchronize()
     mm.synchronize()
-    start, end = mm.Event(), mm.Event()
-    start.record(stream)
-    plan.spmm_repeat_ptr(B.ptr, C.ptr, N, args.steps, stream, cfg)
-    end.record(stream)
-    stream.synchronize()
-    mm.synchronize()
-    ms = start.elapsed_ms(end) / max(args.steps, 1)
+    launch = lambda: plan.spmm_ptr(B.ptr, C.ptr, N, stream, cfg)
+    # cold: a copy over 2 x flush-mb MiB between launches (outside their
+    # events) leaves none of A, B, C in L2 or the Infinity Cache
+    nb = args.flush_mb << 20
+    fsrc, fdst = mm.DeviceArray((nb,), np.uint8), mm.DeviceArray((nb,), np.uint8)
+    flush = lambda: mm.check(mm.lib().bcsr_copy_kernel(fdst.ptr, fsrc.ptr, nb, stream.handle), "flush")
+    cold_ms, cold_wall = timed_launches(mm, np, launch, stream, args.steps, flush=flush)
+    warm_ms, warm_wall = timed_launches(mm, np, launch, stream, args.steps)
+    fsrc.free()
+    fdst.free()
+    timing = {"cold": {"median_ms": float(np.median(cold_ms)), "mean_ms": float(np.mean(cold_ms)),
+                       "min_ms": float(np.min(cold_ms)), "max_ms": float(np.max(cold_ms)),
+                       "wall_s_incl_flushes": round(cold_wall, 4),
+                       "flush": f"copy of {args.flush_mb} MiB to a second {args.flush_mb} MiB "
+                                f"buffer before each launch, outside its events"},
+              "warm": {"median_ms": float(np.median(warm_ms)), "mean_ms": float(np.mean(warm_ms)),
+                       "min_ms": float(np.min(warm_ms)), "max_ms": float(np.max(warm_ms)),
+                       "wall_s": round(warm_wall, 4),
+                       "note": "back-to-back launches: A, B and C stay in the 256 MB Infinity Cache"}}
+    ms = timing[args.timing]["median_ms"]
     t = ms * 1e-3
-    print(f"# timed {args.steps} steps: {ms * 1e3:.2f} us/step ({kname})", file=sys.stderr,
+    print(f"# timed {args.steps} steps: cold median {timing['cold']['median_ms'] * 1e3:.2f} us, "
+          f"warm median {timing['warm']['median_ms'] * 1e3:.2f} us ({kname})", file=sys.stderr,
           flush=True)
     if args.calib_copy_mb:
         nb = args.calib_copy_mb << 20
@@ -378,10 +497,14 @@ def main():
     if bound == "hbm":
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}
+        other = {k: round(q / (v["median_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k, v in timing.items()}
     else:
         pk = MFMA_PEAK_TFLOPS[args.precision]
         roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": pk, "unit": "TFLOP/s",
                 "frac": round(tflops / pk, 4), "traffic": None}
+        other = {k: round(flops / (v["median_ms"] * 1e-3) / 1e12 / pk, 4) for k, v in timing.items()}
+    roof["timing"] = args.timing
+    roof["frac_cold"], roof["frac_warm"] = other["cold"], other["warm"]
     # the matrix cores' share of their dense peak at this throughput
     roof["mfma_utilisation"] = round(tflops / MFMA_PEAK_TFLOPS[args.precision], 4)
     roof["kernel"] = kname
@@ -408,7 +531,8 @@ def main():
     if not args.no_cpu_baseline:
         import oracle
 
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = args.cpu_threads or host["affinity_cpus"]
         Bf = B16.astype(np.float32)
         data16 = h.data.view(np.uint16)
         times = []
@@ -445,14 +569,40 @@ def main():
         # fp16 storage is held to 1e-3 of |A||B| per entry; the other modes
         # store different values and are held to their own bounds in the tests
         bound = {"f16": 1e-3, "bf16": 2e-2, "hilo": 1e-3, "fp8": 0.25}[args.precision]
+        # the reference's literal dense cpu_matmul_ref (nvbench_main.cu:78-91),
+        # one thread, on the small case (1024^2, 10 % of 16 x 16 blocks,
+        # N = 256): O(M K N), so only there
+        A0 = mm.generate(1024, 1024, 0.9, "blockrandom", bh=16, seed=123)
+        B0 = mm.generate(1024, 256, 0.0, "random", bh=16, seed=456)
+        h0 = mm.bcsr_from_dense(A0, 16, 16)
+        c0 = time.perf_counter()
+        C0 = oracle.matmul_dense_ref(A0.view(np.uint16), B0.view(np.uint16))
+        td = time.perf_counter() - c0
+        c0 = time.perf_counter()
+        C0b = oracle.bcsr_spmm_f32(h0.hdr, h0.idx, h0.data.view(np.uint16), 16, 16, B0.astype(np.float32),
+                                   threads=1)
+        tb = time.perf_counter() - c0
+        assert np.array_equal(C0.astype(np.float32), C0b), "dense and BCSR-order CPU products differ"
+        f0 = 2.0 * h0.nnzb * 256 * 256
+        dense_small = {"workload": "1024 x 1024, 10 % of 16 x 16 blocks (nnzb "
+                                   f"{h0.nnzb}), N = 256, seeds 123 / 456",
+                       "dense_literal": {"seconds": round(td, 4), "cores": 1,
+                                         "effective_tflops": round(f0 / td / 1e12, 6),
+                                         "dense_equivalent_tflops": round(2.0 * 1024 * 1024 * 256 / td / 1e12, 6)},
+                       "bcsr_order": {"seconds": round(tb, 5), "cores": 1,
+                                      "effective_tflops": round(f0 / tb / 1e12, 6)},
+                       "bit_identical": True}
+        del A0, B0, C0, C0b
         cpu = {"value": round(flops / tc / 1e12, 5), "unit": "TFLOPS", "cores": threads,
                "kind": "port",
                "sample": f"full product (all {h.block_rows} block-rows), oracle BCSR-order fp32 "
-                         f"loop over {threads} threads, median of {len(times)} runs "
-                         f"({tc:.3f} s each)",
+                         f"loop (bit-identical to cpu_matmul_ref) over {threads} threads, median "
+                         f"of {len(times)} runs ({tc:.3f} s each)",
                "seconds": round(tc, 4),
+               "host": host,
                "single_core": {"value": round(flops / t1 / 1e12, 5), "unit": "TFLOPS", "cores": 1,
                                "seconds": round(t1, 4)},
+               "reference_dense_loop_small_case": dense_small,
                "gpu_vs_cpu": {"max_err_over_absAabsB": rel, "bound": bound,
                               "exact_zero_where_absAabsB_zero": exact_zero,
                               "ok": rel <= bound and exact_zero},
@@ -463,11 +613,17 @@ def main():
     extensions = None
     if not args.no_extensions and args.precision == "f16":
         t1 = time.time()
-        extensions = {"precision_modes_at_this_workload":
-                      precision_modes(mm, np, h, h32, B, N, stream, ms)}
+        nb = args.flush_mb << 20
+        fsrc, fdst = mm.DeviceArray((nb,), np.uint8), mm.DeviceArray((nb,), np.uint8)
+        flush = lambda: mm.check(mm.lib().bcsr_copy_kernel(fdst.ptr, fsrc.ptr, nb, stream.handle), "flush")
+        ref = {k: timing[k]["median_ms"] for k in ("cold", "warm")}
+        extensions = {"precision_modes": precision_modes(mm, np, N, stream, ref, flush)}
         extensions["library_rows_at_this_workload"] = library_rows(mm, np, h, B, N, stream, ms, C_gpu)
         extensions["reference_published_shapes"] = published_shapes(mm, np, stream)
-        extensions["batched"] = batched(mm, np, args.batch_count, stream)
+        extensions["batched"] = batched(mm, np, args.batch_count, stream,
+                                        {k: flops / (v * 1e-3) / 1e12 for k, v in ref.items()})
+        fsrc.free()
+        fdst.free()
         print(f"# extensions {time.time() - t1:.1f}s", file=sys.stderr, flush=True)
 
     out = {
@@ -478,6 +634,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 6),
+        "timing": timing,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

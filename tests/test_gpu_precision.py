@@ -13,7 +13,8 @@ is checked twice:
      and B (the fp16 input), with each mode's error bound per entry relative
      to (|A|.|B|)_ij: fp16 2^-11, bf16 2^-8 + 2^-8 (A and the B operand are
      rounded), hi+lo 5e-5; FP8 (2^-4 per operand above E4M3's subnormal
-     range) mean <= 2e-2 and max <= 0.25 on log-uniform values.
+     range) max <= 0.095, mean <= 0.0126 and normwise <= 0.054 on
+     log-uniform values (1.5x the measured errors).
 The measured maxima are printed per mode.
 """
 import sys
@@ -76,9 +77,14 @@ def test_precision_mode(mode, bh, N):
     normwise = np.linalg.norm(C - C64) / np.linalg.norm(C64)
     log(f"  {mode}: max rel |A||B| {rel.max():.3e} mean {rel.mean():.3e} normwise {normwise:.3e}")
     if mode == "fp8":
-        # E4M3 keeps 2^-4 relative precision only above its subnormal range,
-        # which log-uniform blocks over 5 decades reach: report, bound loosely
-        assert rel.mean() <= 2e-2 and rel.max() <= 0.25, f"fp8 error vs FP64 {rel.max():.3e}"
+        # E4M3 keeps 2^-4 relative precision per operand (A and B are both
+        # quantised); with 16-64 blocks per row an entry sums few terms, so
+        # its error reaches 5-6e-2 (the full 16384^2 workload, ~51 blocks
+        # per row, averages it down to 1.9e-2).  Bounds: 1.5x the measured
+        # 6.3e-2 / 8.4e-3 / 3.6e-2.
+        assert rel.max() <= 0.095, f"fp8 max error vs FP64 {rel.max():.3e}"
+        assert rel.mean() <= 0.0126, f"fp8 mean error vs FP64 {rel.mean():.3e}"
+        assert normwise <= 0.054, f"fp8 normwise error vs FP64 {normwise:.3e}"
     else:
         assert rel.max() <= BOUND_VS_FP64[mode], f"{mode}: error vs FP64 above bound ({rel.max():.3e})"
     assert np.all(C[~nz] == 0)

@@ -19,7 +19,8 @@
 3. The batched workload in full: the 96 problems of workloads.draw_problems
    (96, 1000) in one persistent launch, fp16 and FP8: every C equal bit for
    bit to its own per-matrix launch with the batch's tile; fp16 problems on
-   sampled rows against the oracle; FP8 against FP64 on all 96.
+   sampled rows against the oracle; FP8 against FP64 on all 96 (max and mean
+   per entry within 1.5x of the measured 1.02e-1 / 3.2e-3).
 """
 import concurrent.futures as cf
 import sys
@@ -212,8 +213,10 @@ def test_full_batch(precision, batch_workload):
     batch = mm.Batch(plans, Bs, Cs)
     log(f"batch {precision}: {len(probs)} problems, items {batch.info.items}, "
         f"{batch.info.flops / 1e9:.1f} GFLOP")
-    batch.run(graph=True)
-    mm.synchronize()
+    stream = mm.Stream()
+    batch.run(stream, graph=True)  # the graph replay a timed run uses
+    stream.synchronize()
+    stream.destroy()
     tiles = batch.tiles
     got = [C.numpy() for C in Cs]
     for i, (pl, B, t) in enumerate(zip(plans, Bs, tiles)):
@@ -239,7 +242,9 @@ def test_full_batch(precision, batch_workload):
                 emax = max(emax, float(rel.max()))
                 means.append(float(rel.mean()))
         log(f"  FP8 vs FP64 over 96 problems: max {emax:.3e}, mean of means {np.mean(means):.3e}")
-        assert np.mean(means) <= 2e-2 and emax <= 0.25
+        # measured on MI355X: max 1.02e-1 (a problem with few blocks per row),
+        # mean of the per-problem means 3.2e-3; bounds 1.5x those
+        assert np.mean(means) <= 4.8e-3 and emax <= 0.153
     batch.close()
     for x in plans:
         x.close()
