@@ -212,6 +212,17 @@ int bcsr_release_device(const int *gpu_hdr);
 /* The plan behind a registered device hdr pointer (owned by the registry). */
 int bcsr_plan_from_device(const int *gpu_hdr, const bcsr_plan **out);
 
+/* The reference launches its kernels on a stream chosen at the call site
+ * (<<<grid, block, smem, stream>>>, or the null stream when omitted); the
+ * entry points below take it as their last argument, which C++ callers may
+ * leave out (the null stream), so a reference call site only drops the
+ * launch configuration. */
+#ifdef __cplusplus
+#define BCSR_STREAM_ARG void *stream = nullptr
+#else
+#define BCSR_STREAM_ARG void *stream
+#endif
+
 /* ---- the reference's entry points ------------------------------------------
  * hdr/idx/data must come from bcsr_copy_to_device (BCSR_ERR_NOT_REGISTERED
  * otherwise); the block shape named by the entry point must match the
@@ -221,7 +232,7 @@ int bcsr_plan_from_device(const int *gpu_hdr, const bcsr_plan **out);
  * ignored (the library balances work itself). */
 #define BCSR_ENTRY(name) \
     int name(const int *hdr, const int *idx, const void *data, const void *B, float *C, \
-             unsigned int M, unsigned int N, void *stream)
+             unsigned int M, unsigned int N, BCSR_STREAM_ARG)
 BCSR_ENTRY(sparseMatrixMulTensor);             /* src/cuda_kernels.cu:301 16x16 */
 BCSR_ENTRY(sparseMatrixMulTensor_v2);          /* src/cuda_kernels.cu:370 16x16 */
 BCSR_ENTRY(sparseMatrixMulTensor_v3);          /* src/cuda_kernels.cu:423 16x16 */
@@ -237,17 +248,17 @@ BCSR_ENTRY(sparseMatrixMulTensor32x16_v2_shared_vectorized); /* src/cuda_kernels
 #undef BCSR_ENTRY
 /* src/cuda_kernels.cu:330, square n x n */
 int sparseMatrixMulTensor1(const int *hdr, const int *idx, const void *data, const void *B,
-                           float *C, unsigned int n, void *stream);
+                           float *C, unsigned int n, BCSR_STREAM_ARG);
 /* src/cuda_kernels.cu:1171, 1227, 1293 */
 int sparseMatrixMulTensorAdaptive(const int *hdr, const int *idx, const void *data, const void *B,
                                   float *C, unsigned int M, unsigned int N,
-                                  unsigned int colBlocksPerRow, void *stream);
+                                  unsigned int colBlocksPerRow, BCSR_STREAM_ARG);
 int sparseMatrixMulTensorAdaptive32x16(const int *hdr, const int *idx, const void *data,
                                        const void *B, float *C, unsigned int M, unsigned int N,
-                                       unsigned int colBlocksPerRow, void *stream);
+                                       unsigned int colBlocksPerRow, BCSR_STREAM_ARG);
 int sparseMatrixMulTensorAdaptive64x16(const int *hdr, const int *idx, const void *data,
                                        const void *B, float *C, unsigned int M, unsigned int N,
-                                       unsigned int colBlocksPerRow, void *stream);
+                                       unsigned int colBlocksPerRow, BCSR_STREAM_ARG);
 
 /* ---- CSR, hybrid and dense comparison ----------------------------------------
  * bcsr_csr_spmm: C (M x N fp32) = A_csr . B (accumulate 0), C = C + (A_csr . B)
@@ -260,13 +271,13 @@ int bcsr_csr_spmm(const int32_t *hdr, const int32_t *idx, const void *data_f16, 
                   int32_t M, int32_t K, int32_t N, int32_t accumulate, void *stream);
 /* src/cuda_kernels.cu:271 (square n x n, overwrites C). */
 int sparseMatrixMult1Co(const int *hdr, const int *idx, const void *data, const void *B, float *C,
-                        unsigned int n, void *stream);
+                        unsigned int n, BCSR_STREAM_ARG);
 /* src/cuda_kernels.cu:287 (square n x n, C += A . B in place, summed from C). */
 int sparseMatrixMult1(const int *hdr, const int *idx, const void *data, const void *B, float *C,
-                      unsigned int n, void *stream);
+                      unsigned int n, BCSR_STREAM_ARG);
 /* C[i] += P[i], count floats; addMatrices (src/cuda_kernels.cu:357) takes n x n. */
 int bcsr_add_into(float *C, const float *P, int64_t count, void *stream);
-int addMatrices(float *C, const float *CPart, unsigned int n, void *stream);
+int addMatrices(float *C, const float *CPart, unsigned int n, BCSR_STREAM_ARG);
 /* The hybrid (HCSR) product of the reference's legacy driver (main.cu:567-590):
  * C = plan . B (the dense blocks) then C += csr . B (the sparse remainder),
  * without a second C buffer. */

@@ -65,10 +65,16 @@ struct bcsr_plan {
     std::vector<void *> ws_retired;
     // host copies of grp_ptr and row_order (work-item construction)
     std::vector<int32_t> h_grp_ptr, h_row_order;
-    // panel schedule (16-row device blocks, fp16 / bf16 only; n_panels = 0
-    // otherwise): chunk ranges per panel of 16 device block-rows and one
-    // record of kPanelRecInts ints per chunk (layout in kernels_panel.hip)
+    // panel schedule of launch variant 3 (16-row device blocks, fp16 / bf16
+    // only; n_panels = 0 otherwise): chunk ranges per panel of 16 device
+    // block-rows and one record of kPanelRecInts ints per chunk (layout in
+    // kernels_panel.hip).  Built by the first variant-3 launch (ensure_panel),
+    // from the host copies of the pair columns and block counts kept for it;
+    // until then d_panel is null.
     int n_panels = 0;
+    std::mutex *panel_mu = nullptr;
+    std::vector<int32_t> h_grp_col, h_nblk;
+    void *d_panel = nullptr;
     int32_t *d_panel_ptr = nullptr;
     int32_t *d_panel_rec = nullptr;
     int64_t panel_chunks = 0;
@@ -128,6 +134,9 @@ bool pair_shape_supported(int precision, int rs, int epl, int variant);
 bool panel_supported(const bcsr_plan *plan);
 int panel_grid(const bcsr_plan *plan, int n_tiles, int map_mode);
 int launch_panel(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N, hipStream_t stream);
+// Builds the plan's panel schedule if it has none yet (a synchronous upload;
+// BCSR_ERR_UNSUPPORTED while `stream` is being captured into a graph).
+int ensure_panel(bcsr_plan *plan, hipStream_t stream);
 void panel_kernel_name(const bcsr_plan *plan, char *buf, size_t len);
 void pair_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);
 void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, size_t len);

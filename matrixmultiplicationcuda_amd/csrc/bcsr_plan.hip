@@ -242,14 +242,9 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
         }
     }
 
-    // panel schedule for 16-row device blocks in the 16-bit single-operand modes
-    std::vector<int32_t> panel_ptr, panel_rec;
-    if (!fp8 && H == 1 && rs == 1 && groups * 1024 <= INT32_MAX && dev_rows > 0) {
-        std::vector<int32_t> nblk(dev_rows);
-        for (int r = 0; r < dev_rows; ++r)
-            nblk[r] = (hdr[r / sub_rows + 1] - hdr[r / sub_rows]) * pieces;
-        build_panel_schedule(grp_ptr, grp_col, nblk, dev_rows, panel_ptr, panel_rec);
-    }
+    // the panel kernel (launch variant 3) runs 16-row device blocks in the
+    // 16-bit single-operand modes; its schedule is built by its first launch
+    const bool panel_ok = !fp8 && H == 1 && rs == 1 && groups * 1024 <= INT32_MAX && dev_rows > 0;
 
     std::vector<int32_t> row_order(dev_rows);
     for (int r = 0; r < dev_rows; ++r) row_order[r] = r;
@@ -272,9 +267,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     const size_t off_mo = align_up(off_ro + sizeof(int32_t) * std::max(dev_rows, 1), A);
     const size_t off_as = align_up(off_mo + sizeof(int32_t) * std::max(dev_rows, 1), A);
     const size_t off_v = align_up(off_as + sizeof(uint32_t) * std::max<size_t>(ascale.size(), 1), A);
-    const size_t off_pp = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
-    const size_t off_pr = align_up(off_pp + sizeof(int32_t) * panel_ptr.size(), A);
-    const size_t total = align_up(off_pr + sizeof(int32_t) * panel_rec.size(), A);
+    const size_t total = align_up(off_v + std::max<size_t>(vals.size(), 16), A);
 
     int dev = 0;
     rc = hip_status(hipGetDevice(&dev));
@@ -304,9 +297,7 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
                   {plan->d_row_order, row_order.data(), sizeof(int32_t) * dev_rows},
                   {plan->d_mix_order, mix_order.data(), sizeof(int32_t) * dev_rows},
                   {base + off_as, ascale.data(), sizeof(uint32_t) * ascale.size()},
-                  {plan->d_vals, vals.data(), vals.size()},
-                  {base + off_pp, panel_ptr.data(), sizeof(int32_t) * panel_ptr.size()},
-                  {base + off_pr, panel_rec.data(), sizeof(int32_t) * panel_rec.size()}};
+                  {plan->d_vals, vals.data(), vals.size()}};
     for (const Copy &c : copies) {
         if (!c.bytes) continue;
         rc = hip_status(hipMemcpy(c.dst, c.src, c.bytes, hipMemcpyHostToDevice));
@@ -316,11 +307,13 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
             return rc;
         }
     }
-    if (!panel_ptr.empty()) {
-        plan->n_panels = static_cast<int>(panel_ptr.size()) - 1;
-        plan->d_panel_ptr = reinterpret_cast<int32_t *>(base + off_pp);
-        plan->d_panel_rec = reinterpret_cast<int32_t *>(base + off_pr);
-        plan->panel_chunks = panel_ptr.back();
+    if (panel_ok) {
+        plan->n_panels = (dev_rows + 15) / 16;
+        plan->h_grp_col = grp_col;
+        plan->h_nblk.resize(dev_rows);
+        for (int r = 0; r < dev_rows; ++r)
+            plan->h_nblk[r] = (hdr[r / sub_rows + 1] - hdr[r / sub_rows]) * pieces;
+        plan->panel_mu = new std::mutex();
     }
     plan->rs = rs;
     plan->dev_rows = dev_rows;
@@ -369,7 +362,11 @@ int launch_spmm(const bcsr_plan *plan, const void *B, float *C, int N, const bcs
     if (rc != BCSR_OK) return rc;
     if (plan->info.precision == BCSR_PREC_FP8_E4M3)
         return launch_fp8(const_cast<bcsr_plan *>(plan), ls, B, C, N, stream);
-    if (ls.variant == 3) return launch_panel(plan, ls, B, C, N, stream);
+    if (ls.variant == 3) {
+        rc = ensure_panel(const_cast<bcsr_plan *>(plan), stream);
+        if (rc != BCSR_OK) return rc;
+        return launch_panel(plan, ls, B, C, N, stream);
+    }
     return launch_pairs(plan, ls, B, C, N, stream);
 }
 
@@ -521,6 +518,43 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     return BCSR_OK;
 }
 
+int ensure_panel(bcsr_plan *plan, hipStream_t stream) {
+    if (!plan->panel_mu) return BCSR_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(*plan->panel_mu);
+    if (plan->d_panel) return BCSR_OK;
+    // an allocation and a synchronous upload cannot be recorded into a graph
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    int rc = hip_status(hipStreamIsCapturing(stream, &cap));
+    if (rc != BCSR_OK) return rc;
+    if (cap != hipStreamCaptureStatusNone) return BCSR_ERR_UNSUPPORTED;
+    std::vector<int32_t> panel_ptr, panel_rec;
+    try {
+        build_panel_schedule(plan->h_grp_ptr, plan->h_grp_col, plan->h_nblk, plan->dev_rows, panel_ptr, panel_rec);
+    } catch (const std::bad_alloc &) {
+        return BCSR_ERR_OUT_OF_MEMORY;
+    }
+    const size_t off_rec = align_up(sizeof(int32_t) * panel_ptr.size(), 256);
+    const size_t bytes = off_rec + sizeof(int32_t) * panel_rec.size();
+    void *d = nullptr;
+    rc = hip_status(hipMalloc(&d, bytes));
+    if (rc != BCSR_OK) return rc;
+    char *base = static_cast<char *>(d);
+    rc = hip_status(hipMemcpy(base, panel_ptr.data(), sizeof(int32_t) * panel_ptr.size(), hipMemcpyHostToDevice));
+    if (rc == BCSR_OK)
+        rc = hip_status(
+            hipMemcpy(base + off_rec, panel_rec.data(), sizeof(int32_t) * panel_rec.size(), hipMemcpyHostToDevice));
+    if (rc != BCSR_OK) {
+        (void)hipFree(d);
+        return rc;
+    }
+    plan->d_panel_ptr = reinterpret_cast<int32_t *>(base);
+    plan->d_panel_rec = reinterpret_cast<int32_t *>(base + off_rec);
+    plan->panel_chunks = panel_ptr.back();
+    plan->info.device_bytes += static_cast<int64_t>(bytes);
+    plan->d_panel = d;  // set last: a plan whose d_panel is set is complete
+    return BCSR_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -571,6 +605,11 @@ int bcsr_plan_create_f32(const int32_t *hdr, const int32_t *idx, const float *da
 int bcsr_plan_destroy(bcsr_plan *plan) {
     if (!plan) return BCSR_OK;
     int rc = hip_status(hipFree(plan->dev_base));
+    if (plan->d_panel) {
+        const int rc2 = hip_status(hipFree(plan->d_panel));
+        if (rc == BCSR_OK) rc = rc2;
+    }
+    delete plan->panel_mu;
     if (plan->d_ws) plan->ws_retired.push_back(plan->d_ws);
     for (void *ws : plan->ws_retired) {
         int rc2 = hip_status(hipFree(ws));

@@ -355,3 +355,24 @@ def test_quantize_b_with_no_rows_is_a_no_op():
     assert mm.lib().bcsr_quantize_b_e4m3(None, 0, 64, None, None, None, None) == 0
     assert mm.lib().bcsr_quantize_b_e4m3(None, 64, 0, None, None, None, None) == 0
     assert mm.lib().bcsr_quantize_b_e4m3(None, 8, 64, None, None, None, None) == -3
+
+
+def test_header_in_c_and_the_reference_call_site_in_cpp(tmp_path):
+    """bcsr_mi355x.h compiles as C (every stream argument explicit) and as
+    C++, where a reference call site keeps its arguments -- half pointers,
+    unsigned M and N -- and drops only the launch configuration (the stream
+    defaults to the null stream)."""
+    c_src = tmp_path / "use.c"
+    c_src.write_text('#include "bcsr_mi355x.h"\n'
+                     'int f(const int *h, const int *i, const void *d, const void *B, float *C) {\n'
+                     '  return sparseMatrixMulTensor_v2(h, i, d, B, C, 16u, 16u, (void *)0);\n}\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{ROOT}/include",
+                    str(c_src)], check=True, capture_output=True)
+    cpp_src = tmp_path / "use.cpp"
+    cpp_src.write_text('#include "bcsr_mi355x.h"\n#include "half_type.h"\n'
+                       'int f(int *h, int *i, half *d, half *B, float *C, unsigned M, unsigned N) {\n'
+                       '  int rc = sparseMatrixMulTensor_v2(h, i, d, B, C, M, N);\n'
+                       '  rc |= sparseMatrixMulTensorAdaptive64x16(h, i, d, B, C, M, N, 2u);\n'
+                       '  return rc | sparseMatrixMulTensor1(h, i, d, B, C, N);\n}\n')
+    subprocess.run(["/opt/rocm/llvm/bin/clang++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                    f"-I{ROOT}/include", str(cpp_src)], check=True, capture_output=True)
