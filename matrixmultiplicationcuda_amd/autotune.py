@@ -2,9 +2,9 @@
 batch, with a persistent timing database.
 
 Per matrix the candidate set is CONFIGS (output tile width x kernel variant
-for the 16-bit modes, tile width for FP8); a configuration the library does
-not build for a plan (bcsr_spmm_kernel_name refuses it) is recorded as
-missing.  Per batch the tuner measures tile-width policies (the batch's
+x XCD mapping for the 16-bit modes, tile width x XCD mapping for FP8); a
+configuration the library does not build for a plan (bcsr_spmm_kernel_name
+refuses it) is recorded as missing.  Per batch the tuner measures tile-width policies (the batch's
 automatic widths, the learned dispatcher's picks, each problem's own fastest
 width restricted to those the persistent kernel is compiled for), each with a
 sweep of the persistent grid (waves per CU) and the work-item size, and keeps
@@ -23,8 +23,11 @@ import numpy as np
 from . import Batch, check, lib, make_cfg
 
 TILES = (32, 64, 128, 256)
-CONFIGS_16 = tuple((t, v) for t in TILES for v in (0, 1))  # (tile_n, variant)
-CONFIGS_FP8 = tuple((t, -1) for t in TILES)
+XCD_MAPS = (0, 1, 2, 3, 4)  # linear, tiles pinned, rows pinned, chunked runs, 2 x 4 grid
+# (tile_n, variant, xcd_map); a combination the library refuses for a plan
+# (say tiles pinned to XCDs for a tile count 8 does not divide) is missing
+CONFIGS_16 = tuple((t, v, m) for t in TILES for v in (0, 1) for m in XCD_MAPS)
+CONFIGS_FP8 = tuple((t, -1, m) for t in TILES for m in XCD_MAPS)
 BATCH_WAVES = (4, 8, 12, 16)
 BATCH_BINS = (32, 64, 128, 256)
 
@@ -34,8 +37,11 @@ def configs_for(precision):
 
 
 def cfg_of(config):
-    tile, variant = config
-    return make_cfg(tile_n=tile, variant=variant, xcd_map=-1, waves_per_block=1)
+    """Launch configuration of a (tile_n, variant[, xcd_map]) tuple (no map:
+    the library's automatic mapping)."""
+    tile, variant = config[0], config[1]
+    xmap = config[2] if len(config) > 2 else -1
+    return make_cfg(tile_n=tile, variant=variant, xcd_map=xmap, waves_per_block=1)
 
 
 def supported_mask(plan, N, configs):
@@ -107,7 +113,8 @@ def best_batch_tile(configs, values, precision_id, dev_bh):
     block height: the variant batch_variant names, a tile width the batch is
     compiled for, a finite value.  0 (the batch's automatic width) if none."""
     best, best_v = 0, np.inf
-    for (tile, variant), v in zip(configs, values):
+    for c, v in zip(configs, values):
+        tile, variant = c[0], c[1]  # the batch has its own queue: any XCD map
         if v is None or not np.isfinite(v) or variant != batch_variant(precision_id, dev_bh, tile):
             continue
         if not lib().bcsr_batch_tile_supported(precision_id, dev_bh, tile):
@@ -163,6 +170,6 @@ def precision_id(precision):
     return PRECISIONS[precision]
 
 
-__all__ = ["CONFIGS_16", "CONFIGS_FP8", "TimingDB", "time_configs", "time_auto", "tune_batch",
+__all__ = ["CONFIGS_16", "CONFIGS_FP8", "XCD_MAPS", "TimingDB", "time_configs", "time_auto", "tune_batch",
            "batch_variant", "best_batch_tile", "batch_tiles_from_times", "tune_batch_policies",
            "configs_for", "cfg_of", "supported_mask", "record", "check"]

@@ -338,7 +338,8 @@ def test_batch_tile_pickers_take_the_fastest_batch_tile(precision):
             # other variants of the same tile are faster still but not run
             # by the batch, and 256 columns (never batched) fastest of all
             t = np.ones(len(configs))
-            for k, (tile, variant) in enumerate(configs):
+            for k, c in enumerate(configs):
+                tile, variant = c[0], c[1]
                 if tile == want:
                     t[k] = 0.5 if variant == autotune.batch_variant(prec, dev_bh, tile) else 0.1
                 if tile == 256:
