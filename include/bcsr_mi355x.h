@@ -35,7 +35,8 @@ enum bcsr_status {
     BCSR_ERR_OUT_OF_MEMORY = -5,  /* device or host allocation failed */
     BCSR_ERR_NOT_REGISTERED = -6, /* device hdr pointer not produced by bcsr_copy_to_device */
     BCSR_ERR_UNSUPPORTED = -7,    /* precision/variant not available for this plan */
-    BCSR_ERR_UNKNOWN_PATTERN = -8 /* generator pattern name not recognised */
+    BCSR_ERR_UNKNOWN_PATTERN = -8, /* generator pattern name not recognised */
+    BCSR_ERR_SCRATCH_LIMIT = -9    /* bcsr_spmm_noalloc: the product needs more than its fixed scratch */
 };
 
 /* Storage precision of A on the device.  B is fp16 and C fp32 in every mode. */
@@ -127,6 +128,21 @@ int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *sc
  * bcsr_spmm's on the B the quantiser was given.  No reference counterpart. */
 int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, float *C, int32_t N,
                  const bcsr_launch_cfg *cfg, void *stream);
+/* bcsr_spmm_cfg that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The 16-bit
+ * modes launch exactly as bcsr_spmm_cfg (variant 3 only once an earlier
+ * launch built its schedule; BCSR_ERR_UNSUPPORTED before).  An FP8 plan
+ * quantises B into a fixed per-GPU scratch of BCSR_NOALLOC_SCRATCH_BYTES
+ * (static device memory each GPU holds until the process ends) instead of
+ * the plan's workspace: align256(K * N) + align256(ceil(N / 256)) +
+ * 4 * (K / 16) * ceil(N / 256) bytes must fit (K * N up to 16 MiB, e.g.
+ * 16384 x 1024), else BCSR_ERR_SCRATCH_LIMIT before any launch.  Calls on
+ * one GPU share that scratch: calls that overlap on one GPU (unordered
+ * streams) race.  Results equal bcsr_spmm_cfg's bit for bit. */
+#define BCSR_NOALLOC_SCRATCH_BYTES ((16u << 20) + (64u << 10))
+int bcsr_spmm_noalloc(const bcsr_plan *plan, const void *B, float *C, int32_t N,
+                      const bcsr_launch_cfg *cfg, void *stream);
 /* Name of the kernel a launch with this configuration runs (for profiling). */
 int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cfg *cfg,
                           char *buf, size_t buflen);

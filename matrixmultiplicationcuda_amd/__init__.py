@@ -37,6 +37,7 @@ STATUS = {
     -6: "BCSR_ERR_NOT_REGISTERED",
     -7: "BCSR_ERR_UNSUPPORTED",
     -8: "BCSR_ERR_UNKNOWN_PATTERN",
+    -9: "BCSR_ERR_SCRATCH_LIMIT",
 }
 
 # The reference's pattern list (nvbench_main.cu:56-64) plus this library's
@@ -193,6 +194,7 @@ def _declare(lib):
     f("bcsr_copy_kernel", i, _P, _P, ctypes.c_size_t, _P)
     f("bcsr_quantize_b_e4m3", i, _P, _I32, _I32, _P, _P, _P, _P)
     f("bcsr_spmm_b8", i, _P, _P, _P, _P, _I32, _P, _P)
+    f("bcsr_spmm_noalloc", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _P)
     f("bcsr_stream_create", i, ctypes.POINTER(_P))
     f("bcsr_stream_destroy", i, _P)
     f("bcsr_stream_synchronize", i, _P)

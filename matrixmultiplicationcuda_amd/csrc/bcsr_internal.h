@@ -122,6 +122,12 @@ struct SpmmArgs {
 };
 
 int hip_status(hipError_t e);
+// The plan's device is current, N % 16 == 0, B and C present and 16-byte
+// aligned (BCSR_OK also for the empty products N == 0 or M == 0).
+int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N);
+// The FP8 mode's two B-quantiser launches (B8, scales and work laid out as
+// for bcsr_quantize_b_e4m3); K > 0.
+int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, float *work, hipStream_t stream);
 int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls);
 int launch_pairs(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N,
                  hipStream_t stream);

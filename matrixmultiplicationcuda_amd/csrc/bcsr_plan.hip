@@ -337,6 +337,8 @@ int build_plan(const int32_t *hdr, const int32_t *idx, const SrcBlocks &src, int
     return BCSR_OK;
 }
 
+}  // namespace
+
 int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N) {
     if (!plan) return BCSR_ERR_INVALID_ARG;
     if (N < 0) return BCSR_ERR_INVALID_ARG;
@@ -351,6 +353,8 @@ int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N) 
     if ((reinterpret_cast<uintptr_t>(B) | reinterpret_cast<uintptr_t>(C)) & 15u) return BCSR_ERR_SHAPE;
     return BCSR_OK;
 }
+
+namespace {
 
 int launch_spmm(const bcsr_plan *plan, const void *B, float *C, int N, const bcsr_launch_cfg *cfg,
                 hipStream_t stream) {
@@ -571,6 +575,7 @@ const char *bcsr_status_string(int status) {
         case BCSR_ERR_NOT_REGISTERED: return "device hdr pointer not registered by copyToDevice";
         case BCSR_ERR_UNSUPPORTED: return "unsupported precision or variant";
         case BCSR_ERR_UNKNOWN_PATTERN: return "unknown pattern";
+        case BCSR_ERR_SCRATCH_LIMIT: return "product exceeds the no-allocation scratch";
         default: return "unknown status";
     }
 }

@@ -24,6 +24,8 @@ __global__ __launch_bounds__(128) void quant_b_fp8(const _Float16 *__restrict__ 
     quant_slab(B, part, B8, scale, N, chunks, slabs, blockIdx.x, blockIdx.y);
 }
 
+}  // namespace
+
 // work: (K / 16) * chunks floats of partial maxima.
 int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, float *work, hipStream_t stream) {
     const int slabs = K / 16, chunks = (N + kChunk - 1) / kChunk;
@@ -34,6 +36,8 @@ int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, flo
                        chunks, slabs);
     return hip_status(hipGetLastError());
 }
+
+namespace {
 
 template <int RS, int EPL>
 __global__ __launch_bounds__(256) void bcsr_spmm_fp8(SpmmArgs a) {
