@@ -329,8 +329,11 @@ def batched(mm, np, count, stream, headline_tflops):
     library's static choice each) against the batched persistent kernel
     (graph replays), fp16 and FP8.  The batch's tile widths come from the
     autotuner, which measures the policies -- the batch's automatic widths,
-    the learned dispatcher's per-matrix picks and (FP8) each matrix's own
-    exhaustive autotune -- each with its grid/item-size sweep."""
+    the learned dispatcher's per-matrix picks and each matrix's own
+    exhaustive autotune -- each with its grid/item-size sweep.  Per matrix,
+    the learned dispatcher's live regret against exhaustive autotuning and
+    its gain over the library's automatic choice are measured on the same
+    problems."""
     import concurrent.futures as cf
 
     from matrixmultiplicationcuda_amd import autotune, learned, workloads
@@ -358,21 +361,35 @@ def batched(mm, np, count, stream, headline_tflops):
         t_seq = time_ms(stream, seq, reps=5)
         disp = learned.Dispatcher(prec)
         policies = {"auto": None, "learned": disp.batch_tiles(plans, h16s, Ns)}
-        # per-matrix launches with the learned dispatcher's configurations
-        cfgs = [autotune.cfg_of(disp.choose(pl, h, N)[0]) for pl, h, N in zip(plans, h16s, Ns)]
+        # each problem on its own: every configuration timed (exhaustive
+        # autotune), the library's automatic choice, and the learned
+        # dispatcher's pick (which autotunes, live, where its predicted
+        # spread is under 5 %)
+        configs = autotune.configs_for(prec)
+        times = [autotune.time_configs(p, B, C, B.shape[1], configs, stream, reps=3)
+                 for p, B, C in zip(plans, Bs, Cs)]
+        t_best = np.array([np.nanmin(t) for t in times])
+        t_auto = np.array([autotune.time_auto(p, B, C, B.shape[1], stream, reps=3)
+                           for p, B, C in zip(plans, Bs, Cs)])
+        picks = [disp.choose(pl, h, N, B, C, stream) for pl, h, N, B, C in zip(plans, h16s, Ns, Bs, Cs)]
+        cfgs = [autotune.cfg_of(c) for c, _ in picks]
+        t_pick = np.array([autotune.time_configs(p, B, C, B.shape[1], [c], stream, reps=3)[0]
+                           for p, B, C, (c, _) in zip(plans, Bs, Cs, picks)])
+        live = {"problems": count, "configs": len(configs),
+                "learned_mean_regret": round(float(np.mean(t_pick / t_best - 1.0)), 4),
+                "learned_p90_regret": round(float(np.percentile(t_pick / t_best - 1.0, 90)), 4),
+                "static_mean_regret": round(float(np.mean(t_auto / t_best - 1.0)), 4),
+                "autotune_fallbacks": int(sum(fb for _, fb in picks)),
+                "learned_gain_over_static_sum_ms": round(float(t_auto.sum() / t_pick.sum()), 4),
+                "model_trained_on": disp.meta.get("db"), "model_configs": len(disp.configs)}
         disp.close()
+        policies["per_matrix_autotuned"] = autotune.batch_tiles_from_times(
+            plans, Ns, times, configs, mm.PRECISIONS[prec])
 
         def seq_learned():
             for pl, B, C, cfg in zip(plans, Bs, Cs, cfgs):
                 pl.spmm_ptr(B.ptr, C.ptr, B.shape[1], stream, cfg)
         t_seq_l = time_ms(stream, seq_learned, reps=5)
-        if prec == "fp8":
-            # per-problem exhaustive autotune of the tiles (each matrix alone)
-            configs = autotune.configs_for("fp8")
-            times = [autotune.time_configs(p, B, C, B.shape[1], configs, stream, reps=3)
-                     for p, B, C in zip(plans, Bs, Cs)]
-            policies["per_matrix_autotuned"] = autotune.batch_tiles_from_times(
-                plans, Ns, times, configs, 2)
         best, table = autotune.tune_batch_policies(plans, Bs, Cs, policies, stream)
         batch = mm.Batch(plans, Bs, Cs, tiles=best["tiles"], waves_per_cu=best["waves_per_cu"],
                          bin_work=best["bin_work"])
@@ -382,7 +399,8 @@ def batched(mm, np, count, stream, headline_tflops):
                  "batched_ms": round(t_b, 4), "batched_tflops": round(flops / t_b / 1e9, 2),
                  "tile_policy": best["policy"], "waves_per_cu": best["waves_per_cu"],
                  "bin_work": best["bin_work"],
-                 "policies_ms": {k: round(v["ms"], 4) for k, v in table.items()}}
+                 "policies_ms": {k: round(v["ms"], 4) for k, v in table.items()},
+                 "per_matrix_dispatch_live": live}
         # error of every batched result against the FP64 product of the
         # generator's values (scipy's BSR product in float64)
         batch.run(stream)

@@ -53,7 +53,9 @@ typedef struct {
     int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 */
     int32_t xcd_map;  /* -1 = auto, 0 = linear block order, 1 = column tiles pinned to XCDs,
                        * 2 = block-rows pinned to XCDs (all tiles of a row on one XCD),
-                       * 3 = tile-major work cut into 8 equal runs, one per XCD (any tile count) */
+                       * 3 = tile-major work cut into 8 equal runs, one per XCD (any tile count),
+                       * 4 = 2 x 4 grid: XCD x takes half the column tiles (x & 1) and a
+                       *     quarter of the block-rows (x >> 1); even tile counts only */
     int32_t variant;  /* -1 = auto; 0 plain, 1 double-buffered, 2 64-bit addressing (16-bit modes);
                        * 3 = panel kernel: B slabs staged in LDS and shared by 16 block-rows
                        * (fp16 / bf16, 16-row device blocks, 128-column tiles only; never

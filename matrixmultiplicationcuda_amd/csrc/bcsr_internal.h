@@ -158,7 +158,10 @@ void fp8_kernel_name(const bcsr_plan *plan, const LaunchShape &ls, char *buf, si
 // is for tile counts that 8 does not divide: the items in tile-major order
 // (rows in the plan's mixed order, so that any run of them, and any CU's
 // share of it, carries about the average work) are cut into 8 equal runs,
-// one per XCD, so each XCD serves at most two column tiles.
+// one per XCD, so each XCD serves at most two column tiles.  map_mode 4 (even
+// tile counts) is a 2 x 4 grid of XCDs: XCD x takes the column tiles of half
+// x & 1 for every fourth slot, starting at x >> 1, the tiles of one row
+// next to each other, so they share that row's A in the XCD's L2.
 __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot) {
     // threadIdx.x >> 6 is the same in every lane of a wave; readfirstlane
     // tells the compiler so, which keeps everything derived from it (row,

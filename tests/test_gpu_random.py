@@ -51,7 +51,7 @@ def test_random_structures_and_launches(seed):
         C = mm.DeviceArray((M, N), np.float32)
         for launch in range(3):
             cfg = None if launch == 0 else mm.make_cfg(
-                tile_n=int(rng.choice([0, 32, 64, 128, 256])), xcd_map=int(rng.choice([-1, 0, 1, 2, 3])),
+                tile_n=int(rng.choice([0, 32, 64, 128, 256])), xcd_map=int(rng.choice([-1, 0, 1, 2, 3, 4])),
                 variant=int(rng.choice([-1, 0, 1, 2])), waves_per_block=int(rng.choice([0, 1, 2, 4])))
             what = (f"seed={seed} case={case_no} launch={launch} bh={h.bh} bw={h.bw} M={M} K={K} N={N} "
                     f"nnzb={h.nnzb}" + ("" if cfg is None else

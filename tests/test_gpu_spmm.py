@@ -81,7 +81,7 @@ def launch_case(bh, long_rows):
 @pytest.mark.parametrize("wpb", [1, 4])
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("tile_n", [32, 64, 128, 256])
-@pytest.mark.parametrize("xcd_map", [0, 1, 2, 3])
+@pytest.mark.parametrize("xcd_map", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("bh", [16, 32, 64])
 def test_launch_configs(tile_n, xcd_map, bh, variant, wpb):
     case, C_cpu, Cabs = launch_case(bh, variant > 0)
