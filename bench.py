@@ -534,11 +534,13 @@ def main():
         with open(traffic_file) as f:
             tr = json.load(f).get(wl_key)
         if tr and tr.get("kernel") == kname:
-            roof["traffic"] = tr["bytes_per_launch"]
+            # the launch the headline times (cold or warm) where both were counted
+            tb = tr.get("bytes_per_launch_" + args.timing, tr["bytes_per_launch"])
+            roof["traffic"] = tb
             # the roofline at the arithmetic intensity the counters measured
             # (flops over the HBM bytes a launch really moves), next to the
             # algorithmic one above
-            ai = flops / tr["bytes_per_launch"]
+            ai = flops / tb
             ceiling = min(MFMA_PEAK_TFLOPS[args.precision], ai * HBM_PEAK_GBS / 1e3)
             roof["at_measured_traffic"] = {"arith_intensity": round(ai, 2), "ceiling_tflops": round(ceiling, 1),
                                            "frac": round(tflops / ceiling, 4)}
