@@ -410,6 +410,7 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
     const double density = static_cast<double>(plan->info.nnzb) /
                            std::max(1.0, static_cast<double>(plan->info.block_rows) * plan->info.block_cols);
     int tile_n = cfg ? cfg->tile_n : 0;
+    bool few_items = false;  // set by the automatic tile choice
     if (tile_n == 0 && cfg && cfg->variant == 3) tile_n = 128;  // the panel kernel's only width
     if (tile_n == 0) {
         // fastest measured on MI355X at 16384 x 16384, 1-20% blocks:
@@ -439,6 +440,18 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
             if (plan->rs == 2 && N > 64 && N <= 128) tile_n = 64;
             if (wide_ok && density >= 0.15) tile_n = 256;
         }
+        // Products with few work items (device block-rows x column tiles
+        // under kFewItems, about 2.5 per CU) leave CUs idle at these widths:
+        // halve the tile until there are enough (or it is 32 wide).  Over the
+        // 3500 generator matrices of the autotuner's timing database (512 to
+        // 16384 rows and columns, 0.5-25 % blocks, N 32-512) this takes the
+        // automatic choice's mean regret against exhaustive autotuning from
+        // 15.2 % to 3.6 % (fp16) and from 6.4 % to 1.8 % (FP8); the 16384^2
+        // products keep their widths.
+        constexpr int64_t kFewItems = 640;
+        few_items = static_cast<int64_t>(plan->dev_rows) * ((N + tile_n - 1) / tile_n) < kFewItems;
+        while (tile_n > 32 && static_cast<int64_t>(plan->dev_rows) * ((N + tile_n - 1) / tile_n) < kFewItems)
+            tile_n /= 2;
     }
     if (tile_n != 32 && tile_n != 64 && tile_n != 128 && tile_n != 256) return BCSR_ERR_INVALID_ARG;
     // automatic choice: the software-pipelined kernel where its register
@@ -486,7 +499,12 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         // us; two 256-column tiles at 20 %: 143 vs 165 us).  Tile counts that
         // 8 does not divide take the chunked runs (N = 384: 41.5 vs 59.1 us
         // in the linear order).
-        if (plan->rs == 4 || (ls.n_tiles <= 2 && tile_n <= 128 && density >= 0.15))
+        // Products with few work items (above) do best with rows pinned in
+        // the 16-bit modes and with the chunked runs in FP8 (the timing
+        // database's most frequent winners there).
+        if (few_items)
+            map = fp8 ? 3 : 2;
+        else if (plan->rs == 4 || (ls.n_tiles <= 2 && tile_n <= 128 && density >= 0.15))
             map = 2;
         else
             map = xcd_ok ? 1 : 3;

@@ -276,8 +276,34 @@ def test_automatic_choices(bh, density, N, prec, want):
         few = mm.generate_bcsr(2048, K, 1.0 - density, pattern, bh=bh, bw=16, seed=5)
         few = mm.HostBCSR(few.hdr, few.idx, few.data.astype(np.float16), few.block_rows, few.block_cols, bh, 16)
         plan = mm.Plan(few, prec)
-        assert plan.kernel_name(N) == "bcsr_spmm_pair<1, 8, true, f16>"
+        # 128 block-rows: at 256 or 128 columns fewer than 640 work items,
+        # so the automatic choice halves the tile to 64 columns
+        assert plan.kernel_name(N) == "bcsr_spmm_pair<1, 4, true, f16>"
         plan.close()
+
+
+@pytest.mark.parametrize("rows,N,prec,want", [
+    (512, 512, "f16", "bcsr_spmm_pair<1, 2, true, f16>"),    # 32 block-rows: 32 columns (512 items)
+    (2560, 256, "f16", "bcsr_spmm_pair<1, 4, true, f16>"),   # 160 block-rows x 4 tiles of 64 = 640 items
+    (2560, 512, "f16", "bcsr_spmm_pair<1, 8, true, f16>"),   # 160 x 4 tiles of 128: enough items
+    (1024, 512, "fp8", "bcsr_spmm_fp8<1, 2>"),     # 64 block-rows: 32 columns (1024 items)
+])
+def test_automatic_tile_for_few_work_items(rows, N, prec, want):
+    """Products with fewer than 640 (device block-row, column tile) items at
+    the usual width take narrower tiles, down to 32 columns; the results
+    stay equal to the oracle's."""
+    h = mm.generate_bcsr(rows, 1024, 0.9, "blockrandom", bh=16, bw=16, seed=7)
+    if prec == "f16":
+        h = mm.HostBCSR(h.hdr, h.idx, h.data.astype(np.float16), h.block_rows, h.block_cols, 16, 16)
+    plan = mm.Plan(h, prec)
+    assert plan.kernel_name(N) == want
+    if prec == "f16":
+        B16 = mm.generate(1024, N, 0.0, "random", bh=16, seed=8)
+        A16 = np.zeros((rows, 1024), np.float16)
+        case = Case(A16, B16, h, 16, 16)
+        C_cpu, Cabs = cpu_reference(case)
+        assert_close(plan.matmul(B16), C_cpu, Cabs, what=want)
+    plan.close()
 
 
 def test_b_beyond_2gib_uses_64bit_addressing():
