@@ -168,7 +168,9 @@ typedef struct bcsr_batch bcsr_batch;
 typedef struct {
     int32_t waves_per_cu; /* persistent waves per CU: 0 = 8 */
     int32_t bin_work;     /* work per queue item, in 16 x 16 x 32 MFMA-tile steps: 0 = auto */
-    int32_t reserved[6];
+    int32_t queues;       /* 0 = auto, 1 = one global queue, 2 = one queue per XCD (column tiles
+                           * pinned to XCDs, idle waves take the other queues' items) */
+    int32_t reserved[5];
 } bcsr_batch_cfg;
 typedef struct {
     int32_t count, items, grid, precision;

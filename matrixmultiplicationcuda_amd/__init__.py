@@ -91,7 +91,7 @@ class bcsr_plan_info(ctypes.Structure):
 
 class bcsr_batch_cfg(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("bin_work", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("queues", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class bcsr_batch_info(ctypes.Structure):
@@ -600,7 +600,7 @@ class Batch:
     plans: Plan objects of one precision; Bs / Cs: DeviceArrays (K_i x N_i
     fp16, M_i x N_i fp32); tiles: per-problem tile widths (None/0 = auto)."""
 
-    def __init__(self, plans, Bs, Cs, tiles=None, waves_per_cu=0, bin_work=0):
+    def __init__(self, plans, Bs, Cs, tiles=None, waves_per_cu=0, bin_work=0, queues=0):
         self._lib = lib()
         n = len(plans)
         if len(Bs) != n or len(Cs) != n:
@@ -613,7 +613,7 @@ class Batch:
         tt = np.array(list(tiles) if tiles is not None else [0] * n, dtype=np.int32)
         if tt.size == 0:
             tt = np.zeros(1, np.int32)
-        cfg = bcsr_batch_cfg(waves_per_cu, bin_work)
+        cfg = bcsr_batch_cfg(waves_per_cu, bin_work, queues)
         self._handle = ctypes.c_void_p()
         check(self._lib.bcsr_batch_create(hp, hb, hc, _ptr(nn), _ptr(tt), n, ctypes.byref(cfg),
                                           ctypes.byref(self._handle)), "bcsr_batch_create")

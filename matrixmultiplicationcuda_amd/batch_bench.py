@@ -2,7 +2,7 @@
 BCSR x dense problems, replayed from a hipGraph, with a fixed policy.
 
     python -m matrixmultiplicationcuda_amd.batch_bench [--precision fp8] [--count 96]
-        [--waves-per-cu 4] [--bin-work 128] [--reps 20]
+        [--waves-per-cu 4] [--bin-work 128] [--queues 2] [--reps 20]
 
 Problems come from workloads.draw_problems (rows and cols log-uniform on
 [512, 16384] rounded to 64, 16/32/64 x 16 blocks, block density U(0.5%, 25%),
@@ -29,6 +29,8 @@ def main(argv=None):
     ap.add_argument("--count", type=int, default=96)
     ap.add_argument("--waves-per-cu", type=int, default=0, help="0 = the library's choice")
     ap.add_argument("--bin-work", type=int, default=128)
+    ap.add_argument("--queues", type=int, default=0,
+                    help="0 = automatic (one global queue), 1 = global, 2 = one per XCD")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args(argv)
     check(lib().bcsr_set_device(0))
@@ -61,13 +63,15 @@ def main(argv=None):
     stream.synchronize()
     per_matrix_ms = e0.elapsed_ms(e1) / 5
 
-    batch = Batch(plans, Bs, Cs, waves_per_cu=args.waves_per_cu, bin_work=args.bin_work)
+    batch = Batch(plans, Bs, Cs, waves_per_cu=args.waves_per_cu, bin_work=args.bin_work,
+                  queues=args.queues)
     ms = batch.time(stream, graph=True, warmup=3, reps=args.reps)
     print(json.dumps({"precision": args.precision, "problems": args.count, "gflop": round(flops / 1e9, 3),
                       "batched_ms": round(ms, 4), "batched_tflops": round(flops / ms / 1e9, 2),
                       "per_matrix_ms": round(per_matrix_ms, 4),
                       "per_matrix_tflops": round(flops / per_matrix_ms / 1e9, 2),
-                      "waves_per_cu": args.waves_per_cu, "bin_work": args.bin_work}), flush=True)
+                      "waves_per_cu": args.waves_per_cu, "bin_work": args.bin_work,
+                      "queues": args.queues}), flush=True)
     batch.close()
     for pl in plans:
         pl.close()

@@ -90,8 +90,10 @@ __device__ __forceinline__ void run_item(const SpmmArgs &a, int key, int tile, i
     }
 }
 
+// The queue counters (kMaxQueues) back to zero before a run.
+constexpr int kMaxQueues = 8;
 __global__ void batch_reset(int *counter) {
-    if (threadIdx.x == 0) *counter = 0;
+    if (threadIdx.x < kMaxQueues) counter[threadIdx.x] = 0;
 }
 
 // One wave per workgroup; every wave leaves once the counter passes n_items.
@@ -103,15 +105,24 @@ __global__ void batch_reset(int *counter) {
 // against 0.554 ms).
 template <int PREC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREC == BPREC_FP8 ? 2 : 1))) void
-bcsr_batch_persistent(const BatchMat *__restrict__ mats,
-                                                            const int4 *__restrict__ items, int n_items,
-                                                            int *counter) {
-    for (;;) {
+bcsr_batch_persistent(const BatchMat *__restrict__ mats, const int4 *__restrict__ items,
+                      const int *__restrict__ qoff, int nq, int *counter) {
+    // queue q holds items qoff[q] .. qoff[q + 1] - 1.  With eight queues a
+    // wave starts at its XCD's (workgroups are dispatched round-robin over
+    // the 8 XCDs) and, once that is empty, takes the others' in turn; it
+    // leaves after finding every queue empty once
+    int q = nq == 1 ? 0 : static_cast<int>(blockIdx.x & 7);
+    for (int left = nq; left > 0;) {
         int it = 0;
-        if (threadIdx.x == 0) it = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) it = atomicAdd(&counter[q], 1);
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
-        if (it >= n_items) return;
-        const int4 w = items[it];
+        const int q0 = qoff[q], q1 = qoff[q + 1];
+        if (it >= q1 - q0) {
+            --left;
+            q = q + 1 == nq ? 0 : q + 1;
+            continue;
+        }
+        const int4 w = items[q0 + it];
         const BatchMat &m = mats[w.x];
         const SpmmArgs a = m.a;
         run_item<PREC>(a, m.key, w.y, w.z, w.w);
@@ -146,7 +157,7 @@ __global__ __launch_bounds__(128) void batch_quant(const QuantJob *__restrict__ 
     }
 }
 
-typedef void (*PersistentKernel)(const BatchMat *, const int4 *, int, int *);
+typedef void (*PersistentKernel)(const BatchMat *, const int4 *, const int *, int, int *);
 
 PersistentKernel pick_persistent(int prec) {
     switch (prec) {
@@ -167,6 +178,7 @@ struct bcsr_batch {
     int precision = 0;
     int count = 0;
     int n_items = 0;
+    int n_queues = 1;
     int grid = 0;
     int n_jobs = 0;
     int quant_wg = 0;
@@ -176,6 +188,7 @@ struct bcsr_batch {
     void *d_base = nullptr;
     BatchMat *d_mats = nullptr;
     int4 *d_items = nullptr;
+    int *d_qoff = nullptr;
     int *d_counter = nullptr;
     QuantJob *d_jobs = nullptr;
     std::vector<int32_t> tiles;
@@ -197,7 +210,8 @@ int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
     if (rc != BCSR_OK) return rc;
     if (!b->n_items) return BCSR_OK;
     PersistentKernel k = pick_persistent(b->precision);
-    hipLaunchKernelGGL(k, dim3(b->grid), dim3(64), 0, stream, b->d_mats, b->d_items, b->n_items, b->d_counter);
+    hipLaunchKernelGGL(k, dim3(b->grid), dim3(64), 0, stream, b->d_mats, b->d_items, b->d_qoff, b->n_queues,
+                       b->d_counter);
     return hip_status(hipGetLastError());
 }
 
@@ -227,6 +241,8 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     if (rc != BCSR_OK) return rc;
     const int wpc = cfg && cfg->waves_per_cu > 0 ? cfg->waves_per_cu : 8;
     if (wpc > 32) return BCSR_ERR_INVALID_ARG;
+    const int qmode = cfg ? cfg->queues : 0;
+    if (qmode < 0 || qmode > 2) return BCSR_ERR_INVALID_ARG;
     const int precision = count ? plans[0]->info.precision : BCSR_PREC_F16;
     const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
 
@@ -344,12 +360,44 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     });
     if (items.size() > static_cast<size_t>(INT32_MAX)) return BCSR_ERR_UNSUPPORTED;
 
+    // Queues.  One global queue, or one per XCD: a problem's column tiles
+    // pinned to XCDs as in the per-matrix kernels (tile t of nt on the XCDs
+    // x with x % nt == t, its runs of rows dealt over them in turn), so each
+    // tile's slice of B is fetched into 8 / nt L2s instead of all eight;
+    // tile counts that do not divide 8 deal their items over the XCDs in
+    // turn.  Every queue keeps the global order.
+    const int nq = qmode == 2 ? kMaxQueues : 1;
+    std::vector<std::vector<int4>> qitems(nq);
+    {
+        int prev_i = -1, prev_t = -1, run = 0, rr = 0;
+        for (const Item &it : items) {
+            const int i = it.w.x, t = it.w.y, nt = mats[i].a.n_tiles;
+            if (i != prev_i || t != prev_t) run = 0;
+            prev_i = i;
+            prev_t = t;
+            int x = 0;
+            if (nq > 1) {
+                if (nt <= kMaxQueues && kMaxQueues % nt == 0)
+                    x = t + nt * (run % (kMaxQueues / nt));
+                else if (nt % kMaxQueues == 0)
+                    x = t % kMaxQueues;
+                else
+                    x = rr++ % kMaxQueues;
+            }
+            ++run;
+            qitems[x].push_back(it.w);
+        }
+    }
+    std::vector<int> qoff(nq + 1, 0);
+    for (int q = 0; q < nq; ++q) qoff[q + 1] = qoff[q] + static_cast<int>(qitems[q].size());
+
     bcsr_batch *b = new (std::nothrow) bcsr_batch();
     if (!b) return BCSR_ERR_OUT_OF_MEMORY;
     const size_t off_m = 0;
     const size_t off_i = align256(off_m + sizeof(BatchMat) * std::max(count, 1));
-    const size_t off_c = align256(off_i + sizeof(int4) * std::max<size_t>(items.size(), 1));
-    const size_t off_j = align256(off_c + sizeof(int));
+    const size_t off_q = align256(off_i + sizeof(int4) * std::max<size_t>(items.size(), 1));
+    const size_t off_c = align256(off_q + sizeof(int) * (kMaxQueues + 1));
+    const size_t off_j = align256(off_c + sizeof(int) * kMaxQueues);
     const size_t off_w = align256(off_j + sizeof(QuantJob) * std::max<size_t>(jobs.size(), 1));
     const size_t total = off_w + ws_total;
     rc = hip_status(hipMalloc(&b->d_base, total));
@@ -371,14 +419,16 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
         jobs[jn].part = part;
         ++jn;
     }
-    std::vector<int4> witems(items.size());
-    for (size_t k = 0; k < items.size(); ++k) witems[k] = items[k].w;
+    std::vector<int4> witems;
+    witems.reserve(items.size());
+    for (const auto &qv : qitems) witems.insert(witems.end(), qv.begin(), qv.end());
     struct Copy {
         size_t off;
         const void *src;
         size_t bytes;
     } copies[] = {{off_m, mats.data(), sizeof(BatchMat) * count},
                   {off_i, witems.data(), sizeof(int4) * witems.size()},
+                  {off_q, qoff.data(), sizeof(int) * qoff.size()},
                   {off_j, jobs.data(), sizeof(QuantJob) * jobs.size()}};
     for (const Copy &c : copies) {
         if (!c.bytes) continue;
@@ -393,6 +443,7 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     b->precision = precision;
     b->count = count;
     b->n_items = static_cast<int>(items.size());
+    b->n_queues = nq;
     b->grid = grid;
     b->n_jobs = static_cast<int>(jobs.size());
     b->quant_wg = quant_wg;
@@ -401,6 +452,7 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     b->bytes = total;
     b->d_mats = reinterpret_cast<BatchMat *>(base + off_m);
     b->d_items = reinterpret_cast<int4 *>(base + off_i);
+    b->d_qoff = reinterpret_cast<int *>(base + off_q);
     b->d_counter = reinterpret_cast<int *>(base + off_c);
     b->d_jobs = reinterpret_cast<QuantJob *>(base + off_j);
     b->tiles = std::move(tiles);

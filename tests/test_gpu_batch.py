@@ -58,13 +58,17 @@ def teardown(plans, Bs, Cs):
         x.close() if hasattr(x, "close") else x.free()
 
 
+@pytest.mark.parametrize("queues", [1, 2])
 @pytest.mark.parametrize("precision", ["f16", "fp8", "bf16", "hilo"])
-def test_batch_equals_per_matrix(precision):
+def test_batch_equals_per_matrix(precision, queues):
+    """One global queue (1) or one per XCD (2): each problem's C is bit-equal
+    to its own per-matrix launch at the same tile width."""
     problems = small_problems(12, seed=7)
-    log(f"batch {precision}: " + ", ".join(f"{p.rows}x{p.cols}/{p.bh}/N{p.N}" for p in problems))
+    log(f"batch {precision} queues {queues}: "
+        + ", ".join(f"{p.rows}x{p.cols}/{p.bh}/N{p.N}" for p in problems))
     mm.check(mm.lib().bcsr_set_device(0))
     plans, Bs, Cs, hosts, Bh = setup(problems, precision)
-    batch = mm.Batch(plans, Bs, Cs)
+    batch = mm.Batch(plans, Bs, Cs, queues=queues)
     tiles = batch.tiles
     log(f"  items {batch.info.items} grid {batch.info.grid} tiles {list(tiles)}")
     batch.run()
@@ -94,7 +98,8 @@ def test_batch_explicit_tiles_and_queue_shapes():
     mm.check(mm.lib().bcsr_set_device(0))
     plans, Bs, Cs, _, _ = setup(problems, "f16")
     ref = None
-    for tiles, wpc, bin_work in [(None, 0, 0), ("min", 1, 1), ("max", 4, 100000), ("min", 16, 64)]:
+    for tiles, wpc, bin_work, queues in [(None, 0, 0, 0), ("min", 1, 1, 1), ("max", 4, 100000, 2),
+                                         ("min", 16, 64, 2), ("min", 1, 1, 2)]:
         if tiles == "min":
             t = [32] * len(problems)
         elif tiles == "max":
@@ -104,8 +109,9 @@ def test_batch_explicit_tiles_and_queue_shapes():
             t = None
         for C in Cs:
             C.fill_bytes(0xFF)
-        batch = mm.Batch(plans, Bs, Cs, tiles=t, waves_per_cu=wpc, bin_work=bin_work)
-        log(f"  tiles {list(batch.tiles)} wpc {wpc} bin {bin_work} items {batch.info.items}")
+        batch = mm.Batch(plans, Bs, Cs, tiles=t, waves_per_cu=wpc, bin_work=bin_work, queues=queues)
+        log(f"  tiles {list(batch.tiles)} wpc {wpc} bin {bin_work} queues {queues} "
+            f"items {batch.info.items}")
         batch.run()
         mm.synchronize()
         want = per_matrix(plans, Bs, batch.tiles)
@@ -159,6 +165,9 @@ def test_batch_errors():
         mm.Batch(plans, Bs, Cs, tiles=[256, 32, 32])
     assert e.value.status == -7
     del big
+    with pytest.raises(mm.BcsrError) as e:  # queue mode out of range
+        mm.Batch(plans, Bs, Cs, queues=3)
+    assert e.value.status == -1
     batch = mm.Batch(plans, Bs, Cs)
     with pytest.raises(mm.BcsrError) as e:  # the null stream cannot be captured
         batch.run(None, graph=True)
