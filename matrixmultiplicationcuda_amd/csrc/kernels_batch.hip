@@ -131,27 +131,21 @@ bcsr_batch_persistent(const BatchMat *__restrict__ mats, const int4 *__restrict_
 
 // FP8: every problem's B quantised by two launches (per-slab absmax, then
 // the chunk scales and the quantisation), workgroups striding over the
-// (job, slab pair, chunk) list.
-__device__ __forceinline__ const QuantJob &find_job(const QuantJob *__restrict__ jobs, int n_jobs, int w) {
-    int lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].wg0 <= w) lo = mid; else hi = mid - 1;
-    }
-    return jobs[lo];
-}
-
-__global__ __launch_bounds__(128) void batch_absmax(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
+// (job, slab pair, chunk) list; job_of[w] names workgroup w's job (one load
+// instead of a search over the jobs' prefix sums).
+__global__ __launch_bounds__(128) void batch_absmax(const QuantJob *__restrict__ jobs, const int *__restrict__ job_of,
+                                                    int total_wg) {
     for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
-        const QuantJob &J = find_job(jobs, n_jobs, w);
+        const QuantJob &J = jobs[job_of[w]];
         const int local = w - J.wg0;
         fp8dev::absmax_slab(J.B, J.part, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
     }
 }
 
-__global__ __launch_bounds__(128) void batch_quant(const QuantJob *__restrict__ jobs, int n_jobs, int total_wg) {
+__global__ __launch_bounds__(128) void batch_quant(const QuantJob *__restrict__ jobs, const int *__restrict__ job_of,
+                                                   int total_wg) {
     for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
-        const QuantJob &J = find_job(jobs, n_jobs, w);
+        const QuantJob &J = jobs[job_of[w]];
         const int local = w - J.wg0;
         fp8dev::quant_slab(J.B, J.part, J.B8, J.scale, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
     }
@@ -191,6 +185,7 @@ struct bcsr_batch {
     int *d_qoff = nullptr;
     int *d_counter = nullptr;
     QuantJob *d_jobs = nullptr;
+    int *d_job_of = nullptr;
     std::vector<int32_t> tiles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -200,8 +195,10 @@ namespace {
 
 int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
     if (b->n_jobs) {
-        hipLaunchKernelGGL(batch_absmax, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
-        hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->n_jobs, b->quant_wg);
+        hipLaunchKernelGGL(batch_absmax, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->d_job_of,
+                           b->quant_wg);
+        hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->d_job_of,
+                           b->quant_wg);
         int rc = hip_status(hipGetLastError());
         if (rc != BCSR_OK) return rc;
     }
@@ -398,7 +395,13 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     const size_t off_q = align256(off_i + sizeof(int4) * std::max<size_t>(items.size(), 1));
     const size_t off_c = align256(off_q + sizeof(int) * (kMaxQueues + 1));
     const size_t off_j = align256(off_c + sizeof(int) * kMaxQueues);
-    const size_t off_w = align256(off_j + sizeof(QuantJob) * std::max<size_t>(jobs.size(), 1));
+    std::vector<int> job_of(quant_wg);
+    for (size_t jn = 0; jn < jobs.size(); ++jn) {
+        const int end = jn + 1 < jobs.size() ? jobs[jn + 1].wg0 : quant_wg;
+        for (int w = jobs[jn].wg0; w < end; ++w) job_of[w] = static_cast<int>(jn);
+    }
+    const size_t off_jo = align256(off_j + sizeof(QuantJob) * std::max<size_t>(jobs.size(), 1));
+    const size_t off_w = align256(off_jo + sizeof(int) * std::max<size_t>(job_of.size(), 1));
     const size_t total = off_w + ws_total;
     rc = hip_status(hipMalloc(&b->d_base, total));
     if (rc != BCSR_OK) {
@@ -429,7 +432,8 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     } copies[] = {{off_m, mats.data(), sizeof(BatchMat) * count},
                   {off_i, witems.data(), sizeof(int4) * witems.size()},
                   {off_q, qoff.data(), sizeof(int) * qoff.size()},
-                  {off_j, jobs.data(), sizeof(QuantJob) * jobs.size()}};
+                  {off_j, jobs.data(), sizeof(QuantJob) * jobs.size()},
+                  {off_jo, job_of.data(), sizeof(int) * job_of.size()}};
     for (const Copy &c : copies) {
         if (!c.bytes) continue;
         rc = hip_status(hipMemcpy(base + c.off, c.src, c.bytes, hipMemcpyHostToDevice));
@@ -455,6 +459,7 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     b->d_qoff = reinterpret_cast<int *>(base + off_q);
     b->d_counter = reinterpret_cast<int *>(base + off_c);
     b->d_jobs = reinterpret_cast<QuantJob *>(base + off_j);
+    b->d_job_of = reinterpret_cast<int *>(base + off_jo);
     b->tiles = std::move(tiles);
     *out = b;
     return BCSR_OK;
