@@ -69,7 +69,7 @@ def main():
     out = {"source": "reference matrix_generator.cpp via oracle/_ref; fp16 by numpy RNE"}
 
     A16, hdr, idx, data = bcsr_case(1024, 1024, 0.9, "blockrandom", 16, 16, 16)
-    out["config0"] = {"M": 1024, "K": 1024, "sparsity": 0.9, "pattern": "blockrandom",
+    out["blockrandom_1024x1024_bs16"] = {"M": 1024, "K": 1024, "sparsity": 0.9, "pattern": "blockrandom",
                       "blocksize": 16, "seed": 123, "nnzb": int(hdr[-1]),
                       "hdr": hdr.tolist(), "idx": idx.tolist(), "sha_data": sha(data),
                       "sha_dense_f16": sha(A16)}

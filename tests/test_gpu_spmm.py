@@ -44,17 +44,17 @@ def log(msg):
     sys.stdout.flush()
 
 
-def test_config0_against_dense_cpu_matmul_ref():
+def test_small_case_against_dense_cpu_matmul_ref():
     """The small reference case: 1024^2 blockrandom sp 0.9 16x16 seed 123, B 1024x256 seed 456."""
     case = make_case(1024, 1024, 256, 0.9, "blockrandom", 16, 16)
     assert case.h.nnzb == 400
-    log("config0 nnzb=400")
+    log("1024^2 blockrandom 16x16 N=256: nnzb=400")
     C = run_plan(case)
     C_dense = oracle.matmul_dense_ref(case.A16.view(np.uint16), case.B16.view(np.uint16))
     C_cpu, Cabs = cpu_reference(case)
     assert np.array_equal(C_dense, C_cpu)  # the oracle's two forms agree bit for bit
-    rel = assert_close(C, C_cpu, Cabs, what="config0")
-    log(f"config0 max err / (|A||B|) = {rel:.3e}")
+    rel = assert_close(C, C_cpu, Cabs, what="1024^2 blockrandom")
+    log(f"1024^2 blockrandom max err / (|A||B|) = {rel:.3e}")
 
 
 @pytest.mark.parametrize("bh,pattern", [(16, "blockrandom"), (32, "pattern32by16"),

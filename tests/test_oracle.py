@@ -116,8 +116,8 @@ def test_unknown_pattern_raises_in_both():
         oracle.ref_generate_matrix(8, 8, 0.5, "largerandom", 16, 1)
 
 
-def test_config0_golden():
-    g = golden()["config0"]
+def test_small_blockrandom_golden():
+    g = golden()["blockrandom_1024x1024_bs16"]
     A = oracle.generate_matrix(1024, 1024, 0.9, "blockrandom", 16, 123).astype(np.float32)
     A16 = oracle.f32_to_f16_bits(A)
     assert sha(A16) == g["sha_dense_f16"]
