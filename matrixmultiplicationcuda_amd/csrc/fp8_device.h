@@ -162,8 +162,9 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, const
 // columns tile + 16 e + j for lane j of each lane group.  The B loads of the
 // row's octets are issued half an octet ahead of the MFMAs that use them
 // (two half-octet buffers), A one octet ahead, so loads stay in flight across
-// octet boundaries.  Past the row's end loads aim at an out-of-range offset
-// (zeros, no traffic), so every path has the same wait counts.
+// octet boundaries.  A load past the row's end aims at an out-of-range
+// offset (zeros, no traffic); with one row set and tiles of 64 columns or
+// more the loop leaves before issuing any (see below).
 template <int RS, int EPL>
 __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_in) {
     // wave-uniform by construction (scalar index chain)
@@ -264,29 +265,71 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         };
         load_a(q0, avA, saA);
         issue_half(lo_c, hi_c, 0, bufX);
-        for (int q = q0; q < q1; q += 2) {
-            // octet q (A in avA)
-            issue_half(lo_c, hi_c, 1, bufY);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_half(0, bufX, avA, saA);
-            __builtin_amdgcn_sched_barrier(0);
-            load_a(q + 1, avB, saB);
-            issue_half(lo_n, hi_n, 0, bufX);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_half(1, bufY, avA, saA);
-            __builtin_amdgcn_sched_barrier(0);
-            advance(q + 2);
-            // octet q + 1 (A in avB); past the row's end its loads read zeros
-            issue_half(lo_c, hi_c, 1, bufY);
-            __builtin_amdgcn_sched_barrier(0);
-            if (q + 1 < q1) compute_half(0, bufX, avB, saB);
-            __builtin_amdgcn_sched_barrier(0);
-            load_a(q + 2, avA, saA);
-            issue_half(lo_n, hi_n, 0, bufX);
-            __builtin_amdgcn_sched_barrier(0);
-            if (q + 1 < q1) compute_half(1, bufY, avB, saB);
-            __builtin_amdgcn_sched_barrier(0);
-            advance(q + 3);
+        if constexpr (RS == 1 && EPL >= 4) {
+            // Each half of the loop body checks, before loading the next octet,
+            // whether the row has one; the row's last octet is finished in the
+            // branch that leaves the loop, so nothing is loaded past the row's
+            // end (such loads would read zeros but still occupy the load path)
+            // and the main path through the loop has no merge of branches.
+            for (int q = q0;; q += 2) {
+                // octet q (A in avA)
+                issue_half(lo_c, hi_c, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(0, bufX, avA, saA);
+                __builtin_amdgcn_sched_barrier(0);
+                if (q + 1 >= q1) {
+                    compute_half(1, bufY, avA, saA);
+                    break;
+                }
+                load_a(q + 1, avB, saB);
+                issue_half(lo_n, hi_n, 0, bufX);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(1, bufY, avA, saA);
+                __builtin_amdgcn_sched_barrier(0);
+                advance(q + 2);
+                // octet q + 1 (A in avB)
+                issue_half(lo_c, hi_c, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(0, bufX, avB, saB);
+                __builtin_amdgcn_sched_barrier(0);
+                if (q + 2 >= q1) {
+                    compute_half(1, bufY, avB, saB);
+                    break;
+                }
+                load_a(q + 2, avA, saA);
+                issue_half(lo_n, hi_n, 0, bufX);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(1, bufY, avB, saB);
+                __builtin_amdgcn_sched_barrier(0);
+                advance(q + 3);
+            }
+        } else {
+            // (with more row sets or narrow tiles the early exits below cost
+            // registers, and so occupancy: the loads past the row's end stay)
+            for (int q = q0; q < q1; q += 2) {
+                // octet q (A in avA)
+                issue_half(lo_c, hi_c, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(0, bufX, avA, saA);
+                __builtin_amdgcn_sched_barrier(0);
+                load_a(q + 1, avB, saB);
+                issue_half(lo_n, hi_n, 0, bufX);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(1, bufY, avA, saA);
+                __builtin_amdgcn_sched_barrier(0);
+                advance(q + 2);
+                // octet q + 1 (A in avB); past the row's end its loads read zeros
+                issue_half(lo_c, hi_c, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                if (q + 1 < q1) compute_half(0, bufX, avB, saB);
+                __builtin_amdgcn_sched_barrier(0);
+                load_a(q + 2, avA, saA);
+                issue_half(lo_n, hi_n, 0, bufX);
+                __builtin_amdgcn_sched_barrier(0);
+                if (q + 1 < q1) compute_half(1, bufY, avB, saB);
+                __builtin_amdgcn_sched_barrier(0);
+                advance(q + 3);
+            }
         }
     }
 

@@ -172,10 +172,11 @@ __device__ __forceinline__ void store_tile(const Acc<RS, EPL, PREC> &acc, float 
 // are never stored), so loads need no predication.  The two columns of the
 // next pair are fetched by a scalar load one step ahead (scalar loads count
 // in lgkmcnt, so waiting for them never waits for B).  PF double-buffers: the loads of pair p + 1 are issued before the
-// MFMAs of pair p, unconditionally -- past the row's end they aim at an
-// offset beyond the buffer (which returns zeros without touching memory) -- so
-// the wait counts are the same on every path and each wait covers exactly one
-// pair's loads.
+// MFMAs of pair p, unconditionally inside the loop, so the wait counts are the
+// same on every path through it and each wait covers exactly one pair's
+// loads; a load past the row's end aims at an offset beyond the buffer (which
+// returns zeros without touching memory).  Up to 128-column tiles the last one
+// or two pairs leave the loop and issue nothing past the row's end.
 template <int RS, int EPL, bool PF, int PREC>
 __device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile_in) {
     // wave-uniform by construction; scalar registers keep the index chain
@@ -235,17 +236,45 @@ __device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile
             cn = col_of(p0 + 1);
             // sched_barrier keeps the compiler from sinking a pair's loads
             // below the previous pair's MFMAs, which would undo the pipeline
-            for (int p = p0; p < p1; p += 2) {
-                issue(p + 1, cn, rawB, avB);
-                cn = col_of(p + 2);
+            if constexpr (EPL <= 8) {
+                // the loop runs while both pairs it loads exist; the last one
+                // or two pairs follow without loads past the row's end (those
+                // read zeros but still occupy the load path: 1.5 % of the
+                // time at 16384^2, 5 %, N = 512)
+                int p = p0;
+                for (; p + 2 < p1; p += 2) {
+                    issue(p + 1, cn, rawB, avB);
+                    cn = col_of(p + 2);
+                    __builtin_amdgcn_sched_barrier(0);
+                    consume<RS, EPL, PREC>(acc, rawA, avA);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(p + 2, cn, rawA, avA);
+                    cn = col_of(p + 3);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (p + 1 < p1) consume<RS, EPL, PREC>(acc, rawB, avB);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                const bool two = p + 1 < p1;
+                if (two) issue(p + 1, cn, rawB, avB);
                 __builtin_amdgcn_sched_barrier(0);
                 consume<RS, EPL, PREC>(acc, rawA, avA);
                 __builtin_amdgcn_sched_barrier(0);
-                issue(p + 2, cn, rawA, avA);
-                cn = col_of(p + 3);
-                __builtin_amdgcn_sched_barrier(0);
-                if (p + 1 < p1) consume<RS, EPL, PREC>(acc, rawB, avB);
-                __builtin_amdgcn_sched_barrier(0);
+                if (two) consume<RS, EPL, PREC>(acc, rawB, avB);
+            } else {
+                // 256-column tiles keep the single loop (the split measured
+                // 1 % slower there)
+                for (int p = p0; p < p1; p += 2) {
+                    issue(p + 1, cn, rawB, avB);
+                    cn = col_of(p + 2);
+                    __builtin_amdgcn_sched_barrier(0);
+                    consume<RS, EPL, PREC>(acc, rawA, avA);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(p + 2, cn, rawA, avA);
+                    cn = col_of(p + 3);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (p + 1 < p1) consume<RS, EPL, PREC>(acc, rawB, avB);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
         }
     }
