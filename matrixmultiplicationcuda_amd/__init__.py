@@ -556,9 +556,11 @@ class Plan:
         return ms.value
 
     def supports(self, N, cfg) -> bool:
-        """Whether a launch configuration is available for this plan and N."""
+        """Whether a launch configuration (None: the automatic one) is
+        available for this plan and N."""
         buf = ctypes.create_string_buffer(256)
-        return self._lib.bcsr_spmm_kernel_name(self._handle, N, ctypes.byref(cfg), buf, 256) == 0
+        return self._lib.bcsr_spmm_kernel_name(self._handle, N, ctypes.byref(cfg) if cfg is not None else None,
+                                               buf, 256) == 0
 
     def spmm(self, B: DeviceArray, C: DeviceArray = None, stream=None, cfg=None) -> DeviceArray:
         """C = A . B on the GPU (B fp16 K x N, C fp32 M x N DeviceArrays)."""

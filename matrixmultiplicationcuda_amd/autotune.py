@@ -38,7 +38,9 @@ def configs_for(precision):
 
 def cfg_of(config):
     """Launch configuration of a (tile_n, variant[, xcd_map]) tuple (no map:
-    the library's automatic mapping)."""
+    the library's automatic mapping); None stays None (the automatic choice)."""
+    if config is None:
+        return None
     tile, variant = config[0], config[1]
     xmap = config[2] if len(config) > 2 else -1
     return make_cfg(tile_n=tile, variant=variant, xcd_map=xmap, waves_per_block=1)
@@ -53,7 +55,7 @@ def time_configs(plan, B, C, N, configs, stream=None, warmup=2, reps=5):
     out = np.full(len(configs), np.nan)
     for k, c in enumerate(configs):
         cfg = cfg_of(c)
-        if plan.supports(N, cfg):
+        if cfg is None or plan.supports(N, cfg):
             out[k] = plan.time_ptr(B.ptr, C.ptr, N, stream, cfg, warmup, reps)
     return out
 

@@ -1,9 +1,11 @@
 """Learned dispatch: an MLP cost model that maps a matrix's sparsity features
-to the predicted log time of every launch configuration, trained on the
-autotuner's own timings with this library's HIP kernels (bcsr_cm_*), and a
-dispatcher that picks the predicted-fastest configuration -- or falls back to
-exhaustive autotuning when the predicted spread between the fastest and the
-slowest candidate is under 5%.
+to the predicted log time of every launch configuration and of the library's
+automatic choice, trained on the autotuner's own timings with this library's
+HIP kernels (bcsr_cm_*), and a dispatcher that picks the predicted-fastest
+configuration -- or falls back to exhaustive autotuning when the predicted
+spread between the fastest and the slowest candidate is under 5%, and keeps
+the automatic choice unless a configuration is predicted faster than it by
+more than a margin chosen on validation matrices.
 
 Features (24, in this order): log2 rows, log2 cols, log2 N, bh / 64,
 bw / 64, block density, log2(nnzb + 1), log2(mean blocks per block-row + 1),
@@ -25,6 +27,10 @@ from . import DeviceArray, HostBCSR, check, lib, _ptr, stream_handle
 N_FEATURES = 24
 HIDDEN = 64
 FALLBACK_SPREAD = 0.05
+AUTO = -1  # choose(): the library's automatic choice
+# margins (predicted gain over the automatic choice) the trainer tries on
+# its validation matrices
+MARGINS = (0.0, 0.01, 0.02, 0.03, 0.05, 0.075, 0.1, 0.15, 0.2, 0.3)
 
 
 def features(h: HostBCSR, N: int) -> np.ndarray:
@@ -173,18 +179,24 @@ class CostModel:
 
 
 def dataset(records):
-    """(features, log-ms targets, support mask, ms, auto_ms) from DB records."""
+    """(features, log-ms targets, support mask, ms, auto_ms) from DB records.
+    Targets and mask have one column per configuration and a last one for
+    the automatic choice (mask 0 where a record has no automatic time)."""
     F = np.array([r["features"] for r in records], np.float32)
     ms = np.array([[np.nan if t is None else t for t in r["ms"]] for r in records], np.float64)
-    M = np.isfinite(ms).astype(np.float32)
-    Y = np.where(M > 0, np.log(np.where(M > 0, ms, 1.0)), 0.0).astype(np.float32)
-    auto = np.array([r.get("auto_ms", np.nan) for r in records], np.float64)
+    auto = np.array([np.nan if r.get("auto_ms") is None else r["auto_ms"] for r in records], np.float64)
+    full = np.concatenate([ms, auto[:, None]], axis=1)
+    M = np.isfinite(full).astype(np.float32)
+    Y = np.where(M > 0, np.log(np.where(M > 0, full, 1.0)), 0.0).astype(np.float32)
     return F, Y, M, ms, auto
 
 
-def choose(pred_log, mask, spread=FALLBACK_SPREAD):
-    """Index of the predicted-fastest supported config, and whether the
-    predicted spread is under `spread` (then the dispatcher autotunes)."""
+def choose(pred_log, mask, spread=FALLBACK_SPREAD, pred_auto=None, margin=0.0):
+    """(k, fallback): k the index of the predicted-fastest supported config,
+    or AUTO when pred_auto (the predicted log time of the library's
+    automatic choice) is given and no supported config is predicted faster
+    than it by more than `margin`; fallback whether the predicted spread of
+    the supported configs is under `spread` (then the dispatcher autotunes)."""
     p = np.where(mask > 0, pred_log, np.inf)
     k = int(np.argmin(p))
     sup = pred_log[mask > 0]
@@ -193,17 +205,37 @@ def choose(pred_log, mask, spread=FALLBACK_SPREAD):
     # spread of predicted times exp(max - min) - 1, compared in log space
     # (a badly trained model can predict log-time gaps far beyond exp's range)
     fallback = bool(sup.size > 1 and float(sup.max() - sup.min()) < math.log1p(spread))
+    if (pred_auto is not None and not fallback and np.isfinite(pred_auto)
+            and not float(p[k]) < float(pred_auto) - math.log1p(margin)):
+        return AUTO, False
     return k, fallback
 
 
-def regret_report(pred_log, M, ms, auto_ms, spread=FALLBACK_SPREAD):
+def split_prediction(pred_log, n_configs):
+    """(per-config predictions, automatic-choice predictions or None)."""
+    pred_log = np.atleast_2d(pred_log)
+    if pred_log.shape[1] == n_configs + 1:
+        return pred_log[:, :n_configs], pred_log[:, n_configs]
+    return pred_log[:, :n_configs], None
+
+
+def regret_report(pred_log, M, ms, auto_ms, spread=FALLBACK_SPREAD, margin=0.0):
     """Regret of learned dispatch (with and without the autotune fallback)
-    and of the library's static choice, against exhaustive autotuning."""
-    best = np.nanmin(ms, axis=1)
-    learned, learned_fb, fallbacks = [], [], 0
+    and of the library's static choice, against exhaustive autotuning (the
+    fastest of the configurations and the automatic choice).
+    pred_log has one column per configuration and, for a model with the
+    automatic choice's output, one more: then the dispatcher keeps the
+    automatic choice unless a config is predicted faster by > margin."""
+    T = ms.shape[1]
+    pc, pa = split_prediction(pred_log, T)
+    # the fastest of everything timed: every configuration and the automatic choice
+    best = np.fmin(np.nanmin(ms, axis=1), auto_ms)
+    learned, learned_fb, fallbacks, kept_auto = [], [], 0, 0
     for i in range(ms.shape[0]):
-        k, fb = choose(pred_log[i], M[i], spread)
-        r = ms[i, k] / best[i] - 1.0
+        k, fb = choose(pc[i], M[i, :T], spread, None if pa is None else pa[i], margin)
+        t = auto_ms[i] if k == AUTO else ms[i, k]
+        kept_auto += k == AUTO
+        r = t / best[i] - 1.0
         learned.append(r)
         if fb:
             fallbacks += 1
@@ -211,13 +243,21 @@ def regret_report(pred_log, M, ms, auto_ms, spread=FALLBACK_SPREAD):
         else:
             learned_fb.append(r)
     static = auto_ms / best - 1.0
-    return {"n": int(ms.shape[0]),
+    return {"n": int(ms.shape[0]), "margin": margin,
             "learned_mean_regret": float(np.mean(learned)),
             "learned_p90_regret": float(np.percentile(learned, 90)),
             "dispatch_mean_regret": float(np.mean(learned_fb)),
             "fallback_fraction": fallbacks / max(ms.shape[0], 1),
+            "kept_automatic_fraction": kept_auto / max(ms.shape[0], 1),
             "static_mean_regret": float(np.nanmean(static)),
             "static_p90_regret": float(np.nanpercentile(static, 90))}
+
+
+def tune_margin(pred_log, M, ms, auto_ms, margins=MARGINS):
+    """The margin with the least mean dispatch regret on (validation)
+    records, the smaller one on ties."""
+    scores = [(regret_report(pred_log, M, ms, auto_ms, margin=m)["dispatch_mean_regret"], m) for m in margins]
+    return min(scores)[1]
 
 
 def default_model_path(precision):
@@ -234,21 +274,24 @@ class Dispatcher:
     def __init__(self, precision="f16", path=None):
         self.precision = precision
         self.model, self.configs, self.meta = CostModel.load(path or default_model_path(precision))
+        self.margin = float(self.meta.get("margin", 0.0))
 
     def predict(self, hosts_and_N):
+        """(per-config log-time predictions, automatic-choice predictions or None)."""
         F = np.stack([features(h, N) for h, N in hosts_and_N])
-        return self.model.predict(F)
+        return split_prediction(self.model.predict(F), len(self.configs))
 
     def choose(self, plan, h, N, B=None, C=None, stream=None):
-        """(config, used_fallback) for one product."""
+        """(config, used_fallback) for one product; config None = the
+        library's automatic choice."""
         from . import autotune
-        pred = self.predict([(h, N)])[0]
+        pc, pa = self.predict([(h, N)])
         mask = autotune.supported_mask(plan, N, self.configs).astype(np.float32)
-        k, fb = choose(pred, mask)
+        k, fb = choose(pc[0], mask, pred_auto=None if pa is None else pa[0], margin=self.margin)
         if fb and B is not None and C is not None:
             t = autotune.time_configs(plan, B, C, N, self.configs, stream)
             k = int(np.nanargmin(t))
-        return self.configs[k], fb
+        return (None if k == AUTO else self.configs[k]), fb
 
     def batch_tiles(self, plans, hosts, Ns):
         """Per-problem tile widths for a batch: the predicted-fastest
@@ -256,7 +299,7 @@ class Dispatcher:
         autotune.batch_variant names for each tile width)."""
         from . import PRECISIONS, autotune
         prec = PRECISIONS[self.precision]
-        pred = self.predict(list(zip(hosts, Ns)))
+        pred, _ = self.predict(list(zip(hosts, Ns)))
         return [autotune.best_batch_tile(self.configs, row, prec, p.info.dev_bh)
                 for p, row in zip(plans, pred)]
 

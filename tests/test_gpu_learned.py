@@ -99,10 +99,11 @@ def test_training_fits_a_smooth_target():
 
 @pytest.mark.parametrize("precision", ["f16", "fp8"])
 def test_learned_dispatch_regret_on_held_out(precision):
-    """Trained on the database's training matrices by the GPU kernels,
-    evaluated on its held-out matrices (disjoint seeds): mean regret of the
-    dispatcher vs exhaustive autotuning <= 10%, and below the library's
-    static choice."""
+    """Trained on the database's training matrices by the GPU kernels (the
+    last 500 of them held back to choose the margin by which a configuration
+    must beat the predicted automatic choice), evaluated on its held-out
+    matrices (disjoint seeds): mean regret of the dispatcher vs exhaustive
+    autotuning <= 10%, and below the library's static choice."""
     from matrixmultiplicationcuda_amd import autotune
     mm.check(mm.lib().bcsr_set_device(0))
     path = os.path.join(DATA, f"timings_{precision}.jsonl")
@@ -112,12 +113,17 @@ def test_learned_dispatch_regret_on_held_out(precision):
     train = [r for r in recs if int(r["key"].split(":")[1]) < 200000]
     test = [r for r in recs if int(r["key"].split(":")[1]) >= 200000]
     assert len(train) >= 500 and len(test) >= 100
-    F, Y, M, ms, auto = learned.dataset(train)
+    train.sort(key=lambda r: int(r["key"].split(":")[1]))
+    fit, val = train[:-500], train[-500:]
+    F, Y, M, ms, auto = learned.dataset(fit)
+    assert Y.shape[1] == ms.shape[1] + 1  # one output per configuration and the automatic choice
     cm = learned.CostModel(24, 64, Y.shape[1], seed=0)
     cm.fit(F, Y, M, steps=3000, lr=3e-3)
+    Fv, _, Mv, msv, autov = learned.dataset(val)
+    margin = learned.tune_margin(cm.predict(Fv), Mv, msv, autov)
     Ft, Yt, Mt, mst, autot = learned.dataset(test)
-    rep = learned.regret_report(cm.predict(Ft), Mt, mst, autot)
-    log(f"{precision} regret report: {rep}")
+    rep = learned.regret_report(cm.predict(Ft), Mt, mst, autot, margin=margin)
+    log(f"{precision} regret report (margin {margin}): {rep}")
     assert rep["dispatch_mean_regret"] <= 0.10
     assert rep["learned_mean_regret"] < rep["static_mean_regret"]
     cm.close()

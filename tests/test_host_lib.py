@@ -310,6 +310,47 @@ def test_dispatcher_choice_edge_cases():
     assert k == 1 and not fb  # a single supported configuration
 
 
+def test_dispatcher_keeps_the_automatic_choice_within_its_margin():
+    """With the automatic choice's predicted time, a configuration is taken
+    only when predicted faster than it by more than the margin; the regret
+    report charges the automatic choice's own time then, and tune_margin
+    picks the margin with the least regret on the records it is given."""
+    import math
+
+    from matrixmultiplicationcuda_amd import learned
+
+    mask = np.array([1, 1, 1], np.float32)
+    pred = np.array([1.0, 1.2, 1.5])
+    k, fb = learned.choose(pred, mask, pred_auto=1.0 + math.log(1.08), margin=0.05)
+    assert k == 0 and not fb  # 8 % faster than automatic, margin 5 %: take config 0
+    k, fb = learned.choose(pred, mask, pred_auto=1.0 + math.log(1.03), margin=0.05)
+    assert k == learned.AUTO and not fb  # only 3 % faster: keep the automatic choice
+    k, fb = learned.choose(pred, mask, pred_auto=0.5, margin=0.0)
+    assert k == learned.AUTO  # automatic predicted fastest
+    k, fb = learned.choose(np.array([1.0, 1.01, 1.02]), mask, pred_auto=2.0, margin=0.0)
+    assert fb and k == 0  # predicted spread under 5 %: autotune, whatever the automatic time
+    k, fb = learned.choose(pred, mask, pred_auto=np.nan, margin=0.0)
+    assert k == 0  # no automatic prediction: the fastest configuration
+
+    # two records, three configs; record 0: config 0 is best (1.0 ms), the
+    # automatic choice takes 1.1 ms; record 1: automatic (1.0 ms) is best
+    ms = np.array([[1.0, 2.0, 3.0], [1.3, 2.0, 3.0]])
+    auto = np.array([1.1, 1.0])
+    M = np.ones((2, 4), np.float32)
+    # the model predicts config 0 about 4 % faster than automatic on both
+    p = np.log(np.array([[1.0, 2.0, 3.0, 1.04], [1.0, 2.0, 3.0, 1.04]]))
+    r0 = learned.regret_report(p, M, ms, auto, margin=0.0)
+    assert r0["learned_mean_regret"] == pytest.approx((0.0 + 0.3) / 2)
+    assert r0["static_mean_regret"] == pytest.approx((0.1 + 0.0) / 2)
+    r5 = learned.regret_report(p, M, ms, auto, margin=0.05)
+    assert r5["learned_mean_regret"] == pytest.approx((0.1 + 0.0) / 2)
+    assert r5["kept_automatic_fraction"] == 1.0
+    assert learned.tune_margin(p, M, ms, auto, margins=(0.0, 0.05, 0.1)) == 0.05
+    # a model without the automatic output: the fastest configuration always
+    r = learned.regret_report(p[:, :3], M, ms, auto)
+    assert r["learned_mean_regret"] == pytest.approx(0.15) and r["kept_automatic_fraction"] == 0.0
+
+
 class _StubInfo:
     def __init__(self, dev_bh):
         self.dev_bh = dev_bh
