@@ -113,15 +113,18 @@ int bcsr_spmm_repeat(const bcsr_plan *plan, const void *B, float *C, int32_t N,
 int bcsr_spmm_time(const bcsr_plan *plan, const void *B, float *C, int32_t N,
                    const bcsr_launch_cfg *cfg, int32_t warmup, int32_t reps, void *stream,
                    float *ms);
-/* The FP8 mode's B quantiser, on its own: B (device, K x N fp16) ->
- * B8 (device, K * N bytes, 16-byte aligned), stored k-minor: byte
- * (s * N + n) * 16 + r holds E4M3(B[16 s + r][n] / scale) -- and scales
- * (device, ceil(N / 256) bytes): the E8M0 exponent of each 256-column chunk
- * over all K rows (smallest power of two with |B| / scale <= 448).  work:
- * device scratch of (K / 16) * ceil(N / 256) floats.  B and B8 16-byte
- * aligned, N % 16 == 0. */
+/* The FP8 mode's B quantiser, on its own (one kernel, one pass over B):
+ * B (device, K x N fp16) -> B8 (device, K * N bytes), stored k-minor: byte
+ * (s * N + n) * 16 + r holds E4M3(B[16 s + r][n] / scale(s, n / 256)) -- and
+ * scales (device, bcsr_b8_scale_bytes(K, N) bytes): the E8M0 exponent of each
+ * (16-row slab s, 256-column chunk h), the smallest power of two with
+ * |B| / scale <= 448 over that slab's 16 x 256 patch, at byte
+ * h * ((K / 16 + 15) & ~15) + s.  work is unused (may be NULL).  B, B8 and
+ * scales 16-byte aligned, K % 16 == 0, N % 16 == 0. */
 int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,
                          void *stream);
+/* Bytes of bcsr_quantize_b_e4m3's scales for a K x N B (0 for K or N < 0). */
+int64_t bcsr_b8_scale_bytes(int32_t K, int32_t N);
 /* The FP8 product on a B quantised beforehand by bcsr_quantize_b_e4m3 (K =
  * the plan's columns, same N): only the product kernel runs, so a caller that
  * multiplies one B by several FP8 plans, or the same B more than once,
@@ -137,9 +140,9 @@ int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, floa
  * launch built its schedule; BCSR_ERR_UNSUPPORTED before).  An FP8 plan
  * quantises B into a fixed per-GPU scratch of BCSR_NOALLOC_SCRATCH_BYTES
  * (static device memory each GPU holds until the process ends) instead of
- * the plan's workspace: align256(K * N) + align256(ceil(N / 256)) +
- * 4 * (K / 16) * ceil(N / 256) bytes must fit (K * N up to 16 MiB, e.g.
- * 16384 x 1024), else BCSR_ERR_SCRATCH_LIMIT before any launch.  Calls on
+ * the plan's workspace: align256(K * N) + bcsr_b8_scale_bytes(K, N) bytes
+ * must fit (K * N up to 16 MiB, e.g. 16384 x 1024), else
+ * BCSR_ERR_SCRATCH_LIMIT before any launch.  Calls on
  * one GPU share that scratch: calls that overlap on one GPU (unordered
  * streams) race.  Results equal bcsr_spmm_cfg's bit for bit. */
 #define BCSR_NOALLOC_SCRATCH_BYTES ((16u << 20) + (64u << 10))

@@ -193,6 +193,7 @@ def _declare(lib):
     f("bcsr_memset_async", i, _P, _I32, ctypes.c_size_t, _P)
     f("bcsr_copy_kernel", i, _P, _P, ctypes.c_size_t, _P)
     f("bcsr_quantize_b_e4m3", i, _P, _I32, _I32, _P, _P, _P, _P)
+    f("bcsr_b8_scale_bytes", ctypes.c_int64, _I32, _I32)
     f("bcsr_spmm_b8", i, _P, _P, _P, _P, _I32, _P, _P)
     f("bcsr_spmm_noalloc", i, _P, _P, _P, _I32, ctypes.POINTER(bcsr_launch_cfg), _P)
     f("bcsr_stream_create", i, ctypes.POINTER(_P))

@@ -109,7 +109,7 @@ struct SpmmArgs {
     const int32_t *row_order;
     const _Float16 *B;       // fp16 B (pair kernels)
     const uint8_t *B8;       // quantised B, k-minor panels (FP8 kernel)
-    const uint8_t *b8scale;  // E8M0 per (16-row slab, 256-column chunk)
+    const uint8_t *b8scale;  // E8M0 per (16-row slab, 256-column chunk), chunk-major
     float *C;
     int N;
     int n_tiles;
@@ -117,6 +117,7 @@ struct SpmmArgs {
     int map_mode;
     int b_bytes;  // bytes of B (or B8) when below 2^31: buffer-load bound
     int b_chunks;  // 256-column chunks of B8 scales
+    int b_slabs;   // 16-row slabs of B8 (K / 16)
     int wpb;       // waves per workgroup
     int a_bytes;   // bytes of the FP8 A operands (buffer-load bound)
 };
@@ -125,9 +126,12 @@ int hip_status(hipError_t e);
 // The plan's device is current, N % 16 == 0, B and C present and 16-byte
 // aligned (BCSR_OK also for the empty products N == 0 or M == 0).
 int check_operands(const bcsr_plan *plan, const void *B, const float *C, int N);
-// The FP8 mode's two B-quantiser launches (B8, scales and work laid out as
-// for bcsr_quantize_b_e4m3); K > 0.
-int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, float *work, hipStream_t stream);
+// The FP8 mode's one-pass B quantiser (B8 and scales laid out as for
+// bcsr_quantize_b_e4m3); K > 0.
+int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, hipStream_t stream);
+// Bytes of the B scales of a K x N FP8 product: one per (16-row slab,
+// 256-column chunk), each chunk's row padded to 16 bytes.
+size_t b8_scale_bytes(int K, int N);
 int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, LaunchShape &ls);
 int launch_pairs(const bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, int N,
                  hipStream_t stream);

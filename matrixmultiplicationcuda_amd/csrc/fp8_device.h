@@ -12,12 +12,16 @@
 // Layout that follows: an octet is 8 consecutive 16 x 16 device blocks of a
 // device block-row; block slot s sits in lane group s & 3, byte half s >> 2,
 // so slots (2 t, 2 t + 1) form scale group t and share one A scale (the
-// scale of a 16 x 32 tile).  B is quantised per call with one scale per
-// 256-column chunk (uniform over K), so every scale group of the B operand
-// has the same scale and any two slabs can share a group.  B is stored
-// k-minor, B8[slab][n][16]: the 16 rows of slab c for column n are 16
-// consecutive bytes, exactly one lane's half operand.  All 64 lanes carry
-// data: one octet step moves 8 blocks through one MFMA per 16 columns.
+// scale of a 16 x 32 tile).  B is quantised per call in one pass, with one
+// E8M0 scale per (16-row slab, 256-column chunk), stored chunk-major with
+// rows padded to 16 bytes: bscale[chunk * scale_stride(slabs) + slab].  Where every slab of a chunk has the same
+// scale (checked per row at its start), any two slabs can share a scale group
+// and an octet is one MFMA per 16 columns; otherwise each octet takes two,
+// one over its even slots (A's odd lane groups zeroed) with the even slabs'
+// scales and one over its odd slots with theirs.  B is stored k-minor,
+// B8[slab][n][16]: the 16 rows of slab c for column n are 16 consecutive
+// bytes, exactly one lane's half operand.  All 64 lanes carry data: one octet
+// step moves 8 blocks through one MFMA per 16 columns.
 #pragma once
 
 #include "bcsr_internal.h"
@@ -25,6 +29,9 @@
 namespace fp8dev {
 
 constexpr int kChunk = 256;  // columns per B scale
+
+// Bytes from one chunk's slab scales to the next (16-byte aligned rows).
+__host__ __device__ constexpr int scale_stride(int slabs) { return (slabs + 15) & ~15; }
 
 // Smallest power of two 2^k (k in [-127, 127]) with absmax / 2^k <= 448, as
 // the E8M0 byte k + 127; 127 for an all-zero group.
@@ -83,12 +90,20 @@ __device__ __forceinline__ float block_max128(float v, float *wmax) {
     return v;
 }
 
-// Pass 1: absmax of (slab, chunk h) for slabs 2 x, 2 x + 1 -> part[slab * chunks + h].
-__device__ __forceinline__ void absmax_slab(const _Float16 *__restrict__ B, float *__restrict__ part, int N,
-                                            int chunks, int slabs, int x, int h) {
+// One pass: slabs 2 x, 2 x + 1 of chunk h, each with its own scale (the
+// absmax of its 16 rows x 256 columns) in bscale[h * scale_stride + slab].  Each
+// thread turns its 4 rows x 8 columns into 8 dwords (4 rows of one column
+// each) and puts them in shared memory in the k-minor record order (column
+// record p(c) = c ^ ((c >> 3) & 7) spreads the writes over the banks); thread
+// t then writes the 16-byte records of columns t and t + 128, so the
+// workgroup's global stores are contiguous.
+__device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, uint8_t *__restrict__ B8,
+                                           uint8_t *__restrict__ scale, int N, int slabs, int x, int h) {
     __shared__ float wmax[2];
+    __shared__ unsigned rec[2][kChunk * 4];
     SlabPart sp;
     load_slab_part(B, N, slabs, x, h, sp);
+    float inv[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
         float m = 0.0f;
@@ -97,46 +112,20 @@ __device__ __forceinline__ void absmax_slab(const _Float16 *__restrict__ B, floa
 #pragma unroll
             for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(half_at(sp.v[u][k], e)));
         m = block_max128(m, wmax);
-        if (threadIdx.x == 0 && (u == 0 || sp.ok1)) part[static_cast<size_t>(2 * x + u) * chunks + h] = m;
+        const unsigned se = e8m0_for_absmax(m);
+        if (threadIdx.x == 0 && (u == 0 || sp.ok1))
+            scale[static_cast<size_t>(h) * scale_stride(slabs) + 2 * x + u] = static_cast<uint8_t>(se);
+        inv[u] = ldexpf(1.0f, 127 - static_cast<int>(se));
     }
-}
-
-// Pass 2: slabs 2 x, 2 x + 1 of chunk h quantised.  The chunk's scale comes
-// from every slab's partial (the slab-pair 0 workgroup stores it).  Each
-// thread turns its 4 rows x 8 columns into 8 dwords (4 rows of one column
-// each) and puts them in shared memory in the k-minor record order (column
-// record p(c) = c ^ ((c >> 3) & 7) spreads the writes over the banks); thread
-// t then writes the 16-byte records of columns t and t + 128, so the
-// workgroup's global stores are contiguous.
-__device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, const float *__restrict__ part,
-                                           uint8_t *__restrict__ B8, uint8_t *__restrict__ scale, int N, int chunks,
-                                           int slabs, int x, int h) {
-    __shared__ float wmax[2];
-    __shared__ unsigned rec[2][kChunk * 4];
-    SlabPart sp;
-    load_slab_part(B, N, slabs, x, h, sp);
-    // the chunk's partials: the first 8 per thread as unconditional loads of
-    // a clamped index (all in flight at once), the rest (K > 16384) in a loop
-    float am = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = threadIdx.x + 128 * i;
-        const float v = part[static_cast<size_t>(k < slabs ? k : slabs - 1) * chunks + h];
-        am = fmaxf(am, v);
-    }
-    for (int k = threadIdx.x + 1024; k < slabs; k += 128) am = fmaxf(am, part[static_cast<size_t>(k) * chunks + h]);
-    am = block_max128(am, wmax);
-    const unsigned se = e8m0_for_absmax(am);
-    if (x == 0 && threadIdx.x == 0) scale[h] = static_cast<uint8_t>(se);
-    const float inv = ldexpf(1.0f, 127 - static_cast<int>(se));
     const int g8 = threadIdx.x >> 2;
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            int q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][0], e) * inv, half_at(sp.v[u][1], e) * inv, 0,
-                                                    false);
-            q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][2], e) * inv, half_at(sp.v[u][3], e) * inv, q, true);
+            int q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][0], e) * inv[u], half_at(sp.v[u][1], e) * inv[u],
+                                                    0, false);
+            q = __builtin_amdgcn_cvt_pk_fp8_f32(half_at(sp.v[u][2], e) * inv[u], half_at(sp.v[u][3], e) * inv[u], q,
+                                                true);
             const int col = 8 * g8 + e;
             const int p = col ^ ((col >> 3) & 7);
             rec[u][4 * p + sp.m] = static_cast<unsigned>(q);
@@ -156,6 +145,123 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, const
         }
     }
     __syncthreads();  // rec is reused by the next call
+}
+
+// The scale shared by every slab of chunk h, or -1 if the slabs' scales
+// differ (wave-uniform), in two steps: scale_row_load issues the read of
+// the chunk's first 1024 slab scales (16 per lane, from the chunk's 16-byte
+// aligned row; the buffer bound is the padded row, since buffer loads check
+// bounds per dword), scale_row_uniform compares them (and reads the rest of
+// a longer row).  Lane 0 always holds slab 0; the chunk is uniform if no
+// lane saw a scale other than slab 0's (one ballot, no cross-lane shuffles).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t scale_row_rsrc(const SpmmArgs &a, int h) {
+    const int stride = scale_stride(a.b_slabs);
+    return make_rsrc(a.b8scale + static_cast<size_t>(h) * stride, stride);
+}
+
+__device__ __forceinline__ u32x4 scale_row_load(const SpmmArgs &a, int h) {
+    return __builtin_amdgcn_raw_buffer_load_b128(scale_row_rsrc(a, h), 16 * (threadIdx.x & 63), 0, 0);
+}
+
+__device__ __forceinline__ int scale_row_uniform(const SpmmArgs &a, int h, u32x4 v) {
+    const int slabs = a.b_slabs, lane = threadIdx.x & 63;
+    const int s0 = __builtin_amdgcn_readfirstlane(static_cast<int>(v.x & 0xffu));
+    bool differs = false;
+    for (int base = 0;;) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int sc = static_cast<int>((v[b >> 2] >> (8 * (b & 3))) & 0xffu);
+            differs |= base + 16 * lane + b < slabs && sc != s0;
+        }
+        base += 1024;
+        if (base >= slabs) break;
+        v = __builtin_amdgcn_raw_buffer_load_b128(scale_row_rsrc(a, h), base + 16 * lane, 0, 0);
+    }
+    return __builtin_amdgcn_ballot_w64(differs) ? -1 : s0;
+}
+
+// Work item (br, tile) for a chunk whose slabs have different scales: octet by
+// octet (no pipelining: this is the path of uncommon B), two MFMAs per 16
+// columns -- even slots with the even slabs' scales, odd slots with the odd
+// slabs' -- so every slab keeps its own scale.  Lane group g provides the B
+// scale of scale group g (slots 2 g, 2 g + 1) and holds the data of slots g
+// and g + 4: zeroing A in the odd (even) lane groups leaves only the even
+// (odd) slots in the sum.
+template <int RS, int EPL>
+__device__ __forceinline__ void fp8_work_mixed(const SpmmArgs &a, int br, int tile) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int n_base = tile * (16 * EPL);
+    const unsigned N = static_cast<unsigned>(a.N);
+    const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(a.B8, a.b_bytes);
+    const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
+    const unsigned lane_off = static_cast<unsigned>(n_base + j) * 16u;
+    const uint8_t *sc_chunk = a.b8scale + static_cast<size_t>(n_base / kChunk) * scale_stride(a.b_slabs);
+    f32x4 acc[RS][EPL];
+#pragma unroll
+    for (int s = 0; s < RS; ++s)
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int4 c03 = reinterpret_cast<const int4 *>(a.grp_col)[2 * q];
+        const int4 c47 = reinterpret_cast<const int4 *>(a.grp_col)[2 * q + 1];
+        const int cs[8] = {c03.x, c03.y, c03.z, c03.w, c47.x, c47.y, c47.z, c47.w};
+        // this lane's data slots g, g + 4; its scale group g = slots 2 g, 2 g + 1
+        int cl = cs[0], ch = cs[4], ce = cs[0], co = cs[1];
+#pragma unroll
+        for (int t = 1; t < 4; ++t) {
+            cl = g == t ? cs[t] : cl;
+            ch = g == t ? cs[t + 4] : ch;
+            ce = g == t ? cs[2 * t] : ce;
+            co = g == t ? cs[2 * t + 1] : co;
+        }
+        const int sbe = sc_chunk[ce], sbo = sc_chunk[co];
+        u32x8 av[RS];
+        int sa[RS];
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+            const unsigned aoff = static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32);
+            av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
+            av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
+            sa[s] = static_cast<int>((a.ascale[static_cast<size_t>(q) * RS + s] >> (8 * g)) & 0xffu);
+        }
+        const unsigned lo = static_cast<unsigned>(cl) * (16u * N) + lane_off;
+        const unsigned hi = static_cast<unsigned>(ch) * (16u * N) + lane_off;
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+            const bool ok = n_base + 16 * e + j < a.N;
+            u32x8 b;
+            b.lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ok ? lo + 256u * e : kOutOfRange), 0, 0);
+            b.hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ok ? hi + 256u * e : kOutOfRange), 0, 0);
+            const i32x8 bv = __builtin_bit_cast(i32x8, b);
+#pragma unroll
+            for (int s = 0; s < RS; ++s) {
+#pragma unroll
+                for (int odd = 0; odd < 2; ++odd) {
+                    // A of this pass's lane groups, zeros in the others
+                    const bool keep = (g & 1) == odd;
+                    i32x8 am;
+#pragma unroll
+                    for (int d = 0; d < 8; ++d) am[d] = keep ? static_cast<int>(av[s][d]) : 0;
+                    acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(am, bv, acc[s][e], 0, 0, 0, sa[s],
+                                                                                 0, odd ? sbo : sbe);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+        const int n = n_base + 16 * e + j;
+        if (n >= a.N) continue;
+#pragma unroll
+        for (int s = 0; s < RS; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t row = static_cast<size_t>(br) * (16 * RS) + 16 * s + 4 * g + r;
+                a.C[row * static_cast<size_t>(a.N) + n] = acc[s][e][r];
+            }
+    }
 }
 
 // Work item (device block-row br, tile of 16 * EPL columns).  MFMA e covers
@@ -186,9 +292,14 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 
     const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
     if (q0 < q1) {
-        // B's chunk scale exists only where B has rows (a row with octets
-        // implies K > 0; with K = 0 no quantised B is made)
-        const int sb = a.b8scale[n_base / kChunk];
+        // B's scales exist only where B has rows (a row with octets implies
+        // K > 0; with K = 0 no quantised B is made); sb, the chunk's one
+        // scale, is read below, after the first loads
+        int sb = 0;
+        // the chunk's slab scales, read before the pipeline's first loads so
+        // that waiting for them does not wait for those
+        const u32x4 srow = scale_row_load(a, n_base / kChunk);
+
         // slab offsets of this lane's two halves (slots g and 4 + g) in octet q;
         // the eight columns come in as two scalar 16-byte loads (clamped to the
         // last octet) and are picked per lane group by selects, not branches
@@ -265,6 +376,14 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         };
         load_a(q0, avA, saA);
         issue_half(lo_c, hi_c, 0, bufX);
+        // checked after the pipeline's first loads are issued; a chunk whose
+        // slab scales differ takes the two-MFMA path (the loads issued here
+        // are then unused)
+        sb = scale_row_uniform(a, n_base / kChunk, srow);
+        if (sb < 0) {
+            fp8_work_mixed<RS, EPL>(a, br, tile);
+            return;
+        }
         if constexpr (RS == 1 && EPL >= 4) {
             // Each half of the loop body checks, before loading the next octet,
             // whether the row has one; the row's last octet is finished in the

@@ -11,8 +11,8 @@
 // device counter (one atomic per item) until the queue is empty, and runs each
 // with the per-matrix kernels' own work functions, so the batched results are
 // bit-identical to the per-matrix launches.  A run is two kernels: a one-thread
-// reset of the counter and the persistent kernel (plus, for FP8, two batched
-// B-quantisation kernels first); the sequence can be captured once into a
+// reset of the counter and the persistent kernel (plus, for FP8, a batched
+// B-quantisation kernel first); the sequence can be captured once into a
 // hipGraph and replayed.  Memory is never cleared with hipMemsetAsync, so the
 // captured graph holds kernel nodes only.
 #include <algorithm>
@@ -36,11 +36,10 @@ struct QuantJob {
     const _Float16 *B;
     uint8_t *B8;
     uint8_t *scale;
-    float *part;  // per (slab, chunk) absmax
     int N;
     int chunks;
     int slabs;
-    int wg0;  // first workgroup index of this job (prefix over slabs * chunks)
+    int wg0;  // first workgroup index of this job (prefix over slab pairs * chunks)
 };
 
 // Block shapes the persistent kernel is compiled for.  Its register
@@ -129,25 +128,16 @@ bcsr_batch_persistent(const BatchMat *__restrict__ mats, const int4 *__restrict_
     }
 }
 
-// FP8: every problem's B quantised by two launches (per-slab absmax, then
-// the chunk scales and the quantisation), workgroups striding over the
-// (job, slab pair, chunk) list; job_of[w] names workgroup w's job (one load
-// instead of a search over the jobs' prefix sums).
-__global__ __launch_bounds__(128) void batch_absmax(const QuantJob *__restrict__ jobs, const int *__restrict__ job_of,
-                                                    int total_wg) {
-    for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
-        const QuantJob &J = jobs[job_of[w]];
-        const int local = w - J.wg0;
-        fp8dev::absmax_slab(J.B, J.part, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
-    }
-}
-
+// FP8: every problem's B quantised by one launch (each slab with its own
+// scale), workgroups striding over the (job, slab pair, chunk) list;
+// job_of[w] names workgroup w's job (one load instead of a search over the
+// jobs' prefix sums).
 __global__ __launch_bounds__(128) void batch_quant(const QuantJob *__restrict__ jobs, const int *__restrict__ job_of,
                                                    int total_wg) {
     for (int w = blockIdx.x; w < total_wg; w += gridDim.x) {
         const QuantJob &J = jobs[job_of[w]];
         const int local = w - J.wg0;
-        fp8dev::quant_slab(J.B, J.part, J.B8, J.scale, J.N, J.chunks, J.slabs, local / J.chunks, local % J.chunks);
+        fp8dev::quant_slab(J.B, J.B8, J.scale, J.N, J.slabs, local / J.chunks, local % J.chunks);
     }
 }
 
@@ -195,8 +185,6 @@ namespace {
 
 int batch_enqueue(const bcsr_batch *b, hipStream_t stream) {
     if (b->n_jobs) {
-        hipLaunchKernelGGL(batch_absmax, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->d_job_of,
-                           b->quant_wg);
         hipLaunchKernelGGL(batch_quant, dim3(b->quant_grid), dim3(128), 0, stream, b->d_jobs, b->d_job_of,
                            b->quant_wg);
         int rc = hip_status(hipGetLastError());
@@ -287,10 +275,10 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
         if (fp8 && n && p->info.cols) {
             const int slabs = p->info.cols / 16, chunks = (n + 255) / 256;
             ws_off[i] = ws_total;
-            ws_total += align256(static_cast<size_t>(kn)) + align256(static_cast<size_t>(chunks)) +
-                        align256(sizeof(float) * static_cast<size_t>(slabs) * chunks);
+            ws_total += align256(static_cast<size_t>(kn)) + align256(b8_scale_bytes(p->info.cols, n));
             a.b_bytes = static_cast<int>(kn);
             a.b_chunks = chunks;
+            a.b_slabs = slabs;
             a.a_bytes = static_cast<int>(p->info.dev_pairs * p->rs * 2048);
             QuantJob J{};
             J.B = static_cast<const _Float16 *>(B[i]);
@@ -414,12 +402,10 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
         if (!mats[i].a.b_chunks) continue;
         uint8_t *B8 = reinterpret_cast<uint8_t *>(base + off_w + ws_off[i]);
         uint8_t *sc = B8 + align256(static_cast<size_t>(plans[i]->info.cols) * N[i]);
-        float *part = reinterpret_cast<float *>(sc + align256(static_cast<size_t>(jobs[jn].chunks)));
         mats[i].a.B8 = B8;
         mats[i].a.b8scale = sc;
         jobs[jn].B8 = B8;
         jobs[jn].scale = sc;
-        jobs[jn].part = part;
         ++jn;
     }
     std::vector<int4> witems;

@@ -11,29 +11,24 @@ namespace {
 
 using namespace fp8dev;
 
-// B quantiser, grid (ceil(slabs / 2), chunks) x 128 threads: per-slab
-// absmax, then the chunk scales and the quantisation.
-__global__ __launch_bounds__(128) void quant_b_absmax(const _Float16 *__restrict__ B, float *__restrict__ part, int N,
-                                                      int chunks, int slabs) {
-    absmax_slab(B, part, N, chunks, slabs, blockIdx.x, blockIdx.y);
-}
-
-__global__ __launch_bounds__(128) void quant_b_fp8(const _Float16 *__restrict__ B, const float *__restrict__ part,
-                                                   uint8_t *__restrict__ B8, uint8_t *__restrict__ scale, int N,
-                                                   int chunks, int slabs) {
-    quant_slab(B, part, B8, scale, N, chunks, slabs, blockIdx.x, blockIdx.y);
+// B quantiser, grid (ceil(slabs / 2), chunks) x 128 threads: each slab's
+// scale from its own absmax, then its quantised records.
+__global__ __launch_bounds__(128) void quant_b_fp8(const _Float16 *__restrict__ B, uint8_t *__restrict__ B8,
+                                                   uint8_t *__restrict__ scale, int N, int slabs) {
+    quant_slab(B, B8, scale, N, slabs, blockIdx.x, blockIdx.y);
 }
 
 }  // namespace
 
-// work: (K / 16) * chunks floats of partial maxima.
-int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, float *work, hipStream_t stream) {
+size_t b8_scale_bytes(int K, int N) {
+    return static_cast<size_t>(scale_stride(K / 16)) * static_cast<size_t>((N + kChunk - 1) / kChunk);
+}
+
+int enqueue_quant(const void *B, int K, int N, uint8_t *B8, uint8_t *scales, hipStream_t stream) {
     const int slabs = K / 16, chunks = (N + kChunk - 1) / kChunk;
     const dim3 grid((slabs + 1) / 2, chunks);
-    hipLaunchKernelGGL(quant_b_absmax, grid, dim3(128), 0, stream, static_cast<const _Float16 *>(B), work, N, chunks,
+    hipLaunchKernelGGL(quant_b_fp8, grid, dim3(128), 0, stream, static_cast<const _Float16 *>(B), B8, scales, N,
                        slabs);
-    hipLaunchKernelGGL(quant_b_fp8, grid, dim3(128), 0, stream, static_cast<const _Float16 *>(B), work, B8, scales, N,
-                       chunks, slabs);
     return hip_status(hipGetLastError());
 }
 
@@ -93,6 +88,7 @@ int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8
     a.map_mode = ls.map_mode;
     a.b_bytes = static_cast<int>(static_cast<size_t>(plan->info.cols) * N);
     a.b_chunks = (N + kChunk - 1) / kChunk;
+    a.b_slabs = plan->info.cols / 16;
     a.a_bytes = static_cast<int>(std::min<int64_t>(plan->info.dev_pairs * plan->rs * 2048, INT32_MAX));
     a.wpb = ls.wpb;
     hipLaunchKernelGGL(k, dim3(ls.grid), dim3(64 * ls.wpb), 0, stream, a);
@@ -103,12 +99,9 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     if (!pick_kernel(plan->rs, ls.epl)) return BCSR_ERR_UNSUPPORTED;
     const int K = plan->info.cols;
     const int slabs = K / 16;
-    const int chunks = (N + kChunk - 1) / kChunk;
     const size_t q_bytes = static_cast<size_t>(K) * N;
     const size_t off_s = (q_bytes + 255) / 256 * 256;
-    const size_t off_p = off_s + 256 * static_cast<size_t>((chunks + 255) / 256);
-    const size_t work_bytes = sizeof(float) * static_cast<size_t>(slabs) * chunks;
-    const size_t need = off_p + work_bytes;
+    const size_t need = off_s + b8_scale_bytes(K, N);
     std::lock_guard<std::mutex> lk(*plan->ws_mu);
     if (plan->ws_bytes < need) {
         // an allocation cannot be recorded into a graph under capture
@@ -134,11 +127,10 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
     }
     uint8_t *B8 = static_cast<uint8_t *>(plan->d_ws);
     uint8_t *bscale = B8 + off_s;
-    float *part = reinterpret_cast<float *>(B8 + off_p);
     // K = 0: B has no rows to quantise; the product kernel writes C = 0
     // without reading B or its scales
     if (slabs > 0) {
-        int rc = enqueue_quant(B, K, N, B8, bscale, part, stream);
+        int rc = enqueue_quant(B, K, N, B8, bscale, stream);
         if (rc != BCSR_OK) return rc;
     }
     return launch_fp8_product(plan, ls, B8, bscale, C, N, stream);
@@ -146,12 +138,18 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
 
 extern "C" int bcsr_quantize_b_e4m3(const void *B, int32_t K, int32_t N, void *B8, void *scales, void *work,
                                     void *stream) {
+    (void)work;  // the one-pass quantiser needs no scratch
     if (K < 0 || N < 0 || K % 16 || N % 16) return BCSR_ERR_SHAPE;
     if (!K || !N) return BCSR_OK;
-    if (!B || !B8 || !scales || !work) return BCSR_ERR_INVALID_ARG;
+    if (!B || !B8 || !scales) return BCSR_ERR_INVALID_ARG;
     if ((reinterpret_cast<uintptr_t>(B8) & 15u) || (reinterpret_cast<uintptr_t>(B) & 15u) ||
-        (reinterpret_cast<uintptr_t>(work) & 3u))
+        (reinterpret_cast<uintptr_t>(scales) & 15u))
         return BCSR_ERR_SHAPE;
     return enqueue_quant(B, K, N, static_cast<uint8_t *>(B8), static_cast<uint8_t *>(scales),
-                         static_cast<float *>(work), static_cast<hipStream_t>(stream));
+                         static_cast<hipStream_t>(stream));
+}
+
+extern "C" int64_t bcsr_b8_scale_bytes(int32_t K, int32_t N) {
+    if (K < 0 || N < 0) return 0;
+    return static_cast<int64_t>(b8_scale_bytes(K, N));
 }

@@ -3,8 +3,8 @@
 // harnesses that forbid hipMalloc).  No reference counterpart.
 //
 // The only product that needs memory beyond its inputs and outputs is the
-// FP8 mode's quantised B (B8, the chunk scales and the per-slab partial
-// maxima), which bcsr_spmm keeps in a workspace the plan allocates.  Here it
+// FP8 mode's quantised B (B8 and its per-slab scales), which bcsr_spmm keeps
+// in a workspace the plan allocates.  Here it
 // lives in `scratch`, a file-scope device array of kScratchBytes that every
 // GPU holds (once this module is loaded) until the process ends.  Its device
 // address is taken once per device with hipGetSymbolAddress, with that
@@ -13,7 +13,7 @@
 //
 // Every call on one GPU shares that GPU's scratch: calls that overlap on one
 // GPU (different streams, no ordering) race.  Nothing in it needs clearing:
-// the quantiser writes every partial maximum and scale it later reads.
+// the quantiser writes every byte of B8 and every scale the product reads.
 #include <array>
 #include <mutex>
 
@@ -70,18 +70,16 @@ extern "C" int bcsr_spmm_noalloc(const bcsr_plan *plan, const void *B, float *C,
         }
         return launch_pairs(plan, ls, B, C, N, s);
     }
-    const int K = plan->info.cols, slabs = K / 16, chunks = (N + 255) / 256;
+    const int K = plan->info.cols, slabs = K / 16;
     const size_t off_s = align256(static_cast<size_t>(K) * N);
-    const size_t off_p = off_s + align256(static_cast<size_t>(chunks));
-    const size_t need = off_p + sizeof(float) * static_cast<size_t>(slabs) * chunks;
+    const size_t need = off_s + b8_scale_bytes(K, N);
     if (need > kScratchBytes) return BCSR_ERR_SCRATCH_LIMIT;
     uint8_t *base = nullptr;
     rc = scratch_of(plan->device, &base);
     if (rc != BCSR_OK) return rc;
     uint8_t *B8 = base, *bscale = base + off_s;
-    float *part = reinterpret_cast<float *>(base + off_p);
     if (slabs > 0) {
-        rc = enqueue_quant(B, K, N, B8, bscale, part, s);
+        rc = enqueue_quant(B, K, N, B8, bscale, s);
         if (rc != BCSR_OK) return rc;
     }
     return launch_fp8_product(plan, ls, B8, bscale, C, N, s);

@@ -278,15 +278,22 @@ def quant_e4m3_groups(x, group_len):
     return q, sc
 
 
+def b8_scale_stride(K):
+    """Bytes from one chunk's slab scales to the next (rows padded to 16)."""
+    return (K // 16 + 15) & ~15
+
+
 def quant_e4m3_B(B):
-    """(q K x N, scale_exp [ceil(N/256)]) of the B quantiser: one scale per
-    256-column chunk over all rows."""
+    """(q K x N, scale_exp [ceil(N/256), K/16]) of the B quantiser: one scale
+    per (16-row slab, 256-column chunk)."""
     B = np.ascontiguousarray(B, dtype=np.float32)
     K, N = B.shape
+    chunks, stride = (N + 255) // 256, b8_scale_stride(K)
     q = np.empty((K, N), dtype=np.uint8)
-    sc = np.empty(((N + 255) // 256,), dtype=np.uint8)
-    lib().orc_quant_e4m3_B(_p(B), K, N, _p(q), _p(sc))
-    return q, sc
+    sc = np.zeros((chunks, stride), dtype=np.uint8)
+    if K and N:
+        lib().orc_quant_e4m3_B(_p(B), K, N, _p(q), _p(sc))
+    return q, sc[:, :K // 16]
 
 
 def csr_spmm_f32(hdr, idx, data16, Bf32, C0=None):
@@ -381,6 +388,7 @@ def quantized_B(mode, B16_bits):
     if mode == "fp8":
         q, sc = quant_e4m3_B(Bf)
         K, N = Bf.shape
-        scale = np.repeat(e8m0_to_f32(sc), 256)[:N][None, :]
+        # scale of (row k, column n) = that of slab k // 16, chunk n // 256
+        scale = np.repeat(np.repeat(e8m0_to_f32(sc).T, 16, axis=0), 256, axis=1)[:K, :N]
         return e4m3_to_f32(q) * scale
     raise ValueError(mode)

@@ -618,17 +618,19 @@ void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, u
 }
 
 void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale) {
-    int chunks = (N + 255) / 256;
+    const int slabs = K / 16, chunks = (N + 255) / 256, stride = (slabs + 15) & ~15;
     for (int h = 0; h < chunks; ++h) {
         int n0 = h * 256, n1 = n0 + 256 < N ? n0 + 256 : N;
-        float amax = 0.0f;
-        for (int k = 0; k < K; ++k)
-            for (int n = n0; n < n1; ++n) amax = fmaxf(amax, fabsf(B[(size_t)k * N + n]));
-        uint8_t se = orc_e8m0_for_absmax(amax);
-        scale[h] = se;
-        for (int k = 0; k < K; ++k)
-            for (int n = n0; n < n1; ++n)
-                q[(size_t)k * N + n] = orc_f2e4m3(ldexpf(B[(size_t)k * N + n], 127 - (int)se));
+        for (int s = 0; s < slabs; ++s) {
+            float amax = 0.0f;
+            for (int k = 16 * s; k < 16 * s + 16; ++k)
+                for (int n = n0; n < n1; ++n) amax = fmaxf(amax, fabsf(B[(size_t)k * N + n]));
+            uint8_t se = orc_e8m0_for_absmax(amax);
+            scale[(size_t)h * stride + s] = se;
+            for (int k = 16 * s; k < 16 * s + 16; ++k)
+                for (int n = n0; n < n1; ++n)
+                    q[(size_t)k * N + n] = orc_f2e4m3(ldexpf(B[(size_t)k * N + n], 127 - (int)se));
+        }
     }
 }
 

@@ -161,8 +161,9 @@ float orc_e4m32f(uint8_t b);
 uint8_t orc_e8m0_for_absmax(float absmax);
 /* Quantise n groups of `len` floats, one E8M0 scale per group. */
 void orc_quant_e4m3_groups(const float *x, int64_t n, int64_t len, uint8_t *q, uint8_t *scale);
-/* Quantise B (K x N fp32, row-major): one scale per 256-column chunk over
- * all K rows, q row-major like B, scale[ceil(N / 256)]. */
+/* Quantise B (K x N fp32, row-major, K % 16 == 0): one scale per (16-row
+ * slab s, 256-column chunk h), the E8M0 exponent of that 16 x 256 patch's
+ * absmax, at scale[h * ((K / 16 + 15) & ~15) + s]; q row-major like B. */
 void orc_quant_e4m3_B(const float *B, int K, int N, uint8_t *q, uint8_t *scale);
 /* The FP8 mode's A: each 16-row slice of a block-row is a sequence of
  * 16 x 16 device blocks (stored blocks in order, each cut into bw / 16

@@ -160,9 +160,11 @@ int main(int argc, char **argv) {
         const int K = 48, N = 592;  // 3 column chunks, the last partial
         std::vector<float> B(static_cast<size_t>(K) * N);
         for (size_t i = 0; i < B.size(); ++i) B[i] = static_cast<float>((i * 37) % 101) - 50.0f;
-        std::vector<uint8_t> q(B.size()), sc((N + 255) / 256);
+        // one scale per (16-row slab, 256-column chunk), chunk rows padded to 16 bytes
+        std::vector<uint8_t> q(B.size()), sc(static_cast<size_t>((N + 255) / 256) * ((K / 16 + 15) & ~15));
         orc_quant_e4m3_B(B.data(), K, N, q.data(), sc.data());
         for (uint8_t v : q) sum += v;
+        for (uint8_t v : sc) sum += v;
         uint16_t hi, lo;
         orc_hilo_split(1.2345f, &hi, &lo);
         sum += hi + lo;

@@ -216,3 +216,31 @@ def test_fp8_with_empty_inner_dimension():
         x.close()
     for a in (Bdummy, C0, B1, C1):
         a.free()
+
+
+def test_fp8_batch_with_slab_scales_differ():
+    """FP8 batch whose B matrices have 16-row slabs of different magnitudes
+    (each slab keeps its own scale, the two-MFMA path): every C bit-equal to
+    its own per-matrix launch, over two runs."""
+    problems = small_problems(8, seed=23)
+    log("fp8 batch, B slabs scaled: " + ", ".join(f"{p.rows}x{p.cols}/{p.bh}/N{p.N}" for p in problems))
+    mm.check(mm.lib().bcsr_set_device(0))
+    rng = np.random.default_rng(23)
+    plans, Bs, Cs = [], [], []
+    for pr in problems:
+        plans.append(mm.Plan(pr.bcsr(np.float32), "fp8"))
+        B = pr.B().astype(np.float32)
+        B *= np.repeat(2.0 ** rng.integers(-6, 6, pr.cols // 16), 16)[:, None].astype(np.float32)
+        Bs.append(mm.DeviceArray.from_numpy(B.astype(np.float16)))
+        C = mm.DeviceArray((pr.rows, pr.N), np.float32)
+        C.fill_bytes(0xFF)
+        Cs.append(C)
+    batch = mm.Batch(plans, Bs, Cs)
+    want = per_matrix(plans, Bs, batch.tiles)
+    for run in range(2):
+        batch.run()
+        mm.synchronize()
+        for i, (C, w) in enumerate(zip(Cs, want)):
+            assert np.array_equal(C.numpy(), w), f"run {run}: problem {i} ({problems[i]})"
+    batch.close()
+    teardown(plans, Bs, Cs)

@@ -91,28 +91,69 @@ def test_precision_mode(mode, bh, N):
 
 
 def test_fp8_b_quantiser_matches_oracle():
-    """bcsr_quantize_b_e4m3 == the oracle's quantiser, byte for byte."""
+    """bcsr_quantize_b_e4m3 == the oracle's quantiser, byte for byte: one
+    scale per (16-row slab, 256-column chunk), chunk rows 16-byte aligned."""
     lib = mm.lib()
-    K, N = 4160, 640  # 260 slabs (more than one workgroup's stride), 3 chunks, the last partial
+    K, N = 4176, 640  # 261 slabs (an odd last pair, scale rows padded to 272), 3 chunks, the last partial
     B16 = mm.generate(K, N, 0.0, "random", seed=11)
     B16[5, 7] = np.float16(0.0)
-    B16[17, :] *= np.float16(1e-3)  # small values (subnormal codes)
-    B16[4000:4016, 300:310] = np.float16(3.0)  # the absmax of chunk 1 sits in a late slab
+    B16[17, :] *= np.float16(1e-3)  # small values (subnormal codes in slab 1)
+    B16[4000:4016, 300:310] = np.float16(3.0)  # one slab of chunk 1 with a larger scale
+    B16[4160:4176, :256] = np.float16(0.0)  # an all-zero patch (scale 2^0)
     dB = mm.DeviceArray.from_numpy(B16)
     d8 = mm.DeviceArray((K * N,), np.uint8)
-    chunks = (N + 255) // 256
-    ds = mm.DeviceArray((chunks,), np.uint8)
-    work = mm.DeviceArray((K // 16 * chunks,), np.float32)  # per-slab partial maxima
+    stride = oracle.b8_scale_stride(K)
+    assert stride == 272 and lib.bcsr_b8_scale_bytes(K, N) == 3 * stride
+    ds = mm.DeviceArray((3 * stride,), np.uint8)
     want_q, want_s = oracle.quant_e4m3_B(B16.astype(np.float32))
-    for call in range(3):  # repeated calls reuse the scratch
+    assert len(set(want_s[1].tolist())) > 1 and want_s[0, 260] == 127
+    for call in range(3):  # repeated calls
         d8.fill_bytes(0xFF)
         ds.fill_bytes(0)
-        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.ptr, work.ptr, None))
+        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.ptr, None, None))
         mm.synchronize()
         q_k_minor = d8.numpy().reshape(K // 16, N, 16)
         got = q_k_minor.transpose(0, 2, 1).reshape(K, N)
-        assert np.array_equal(ds.numpy(), want_s), f"call {call}"
+        got_s = ds.numpy().reshape(3, stride)
+        assert np.array_equal(got_s[:, :K // 16], want_s), f"call {call}"
         assert np.array_equal(got, want_q), f"call {call}"
+    assert lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.offset(8), None, None) == -3  # misaligned scales
+
+
+@pytest.mark.parametrize("bh,N,tile", [(16, 512, 0), (16, 512, 64), (16, 96, 32), (32, 384, 0), (64, 272, 0),
+                                       (64, 128, 64)])
+def test_fp8_slab_scales_differ(bh, N, tile):
+    """B whose 16-row slabs differ in magnitude (2^-6 .. 2^5) in its first
+    256 columns only: there each slab keeps its own E8M0 scale and the
+    product takes two MFMAs per octet (even slots, odd slots), the other
+    chunk stays on the one-MFMA path.  Against the exact product of the
+    stored values (2^-11, as every FP8 product) and, per entry, within the
+    uniform-B bounds of the FP64 product (the slab scales follow the
+    magnitudes, so the relative error stays that of uniform B)."""
+    M, K = 512, 1024
+    h32, data32, B16 = case(M, K, N, bh, dist="uniform")
+    rng = np.random.default_rng(bh * 1000 + N)
+    mags = (2.0 ** rng.integers(-6, 6, K // 16)).astype(np.float32)
+    B = B16.astype(np.float32)
+    B[:, :256] *= np.repeat(mags, 16)[:, None]
+    B16 = B.astype(np.float16)
+    _, sc = oracle.quant_e4m3_B(B16.astype(np.float32))
+    assert len(set(sc[0].tolist())) > 4
+    assert N <= 256 or len(set(sc[1].tolist())) == 1
+    log(f"bh={bh} N={N} tile={tile} nnzb={h32.nnzb}: chunk 0 slab scales {sorted(set(sc[0].tolist()))}")
+    C = run_mode(h32, B16, "fp8", mm.make_cfg(tile_n=tile) if tile else None).astype(np.float64)
+    Aq = np.asarray(oracle.dequantized_blocks("fp8", data32, bh, 16, h32.hdr, h32.idx), dtype=np.float32)
+    Bq = oracle.quantized_B("fp8", B16.view(np.uint16))
+    Cq, Cqabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, Aq, bh, 16, Bq, threads=8)
+    err_q = np.abs(C - Cq)
+    log(f"  max |C - Cq| / (|Aq||Bq|) {np.max(err_q / np.maximum(Cqabs, 1e-30)):.3e}")
+    assert np.all(err_q <= 2.0**-11 * Cqabs + 1e-30), "kernel differs from its stored values"
+    C64, Cabs = oracle.bcsr_spmm_f64(h32.hdr, h32.idx, data32, bh, 16, B16.astype(np.float32), threads=8)
+    nz = Cabs > 0
+    rel = np.abs(C - C64)[nz] / Cabs[nz]
+    log(f"  max rel |A||B| {rel.max():.3e} mean {rel.mean():.3e}")
+    assert rel.max() <= 0.095 and rel.mean() <= 0.0126
+    assert np.all(C[~nz] == 0)
 
 
 @pytest.mark.parametrize("bh,N", [(16, 512), (32, 96), (64, 272)])
@@ -130,14 +171,12 @@ def test_spmm_on_prequantised_b_equals_spmm(bh, N):
     for n in (N, 48):
         B16 = mm.generate(K, n, 0.0, "random", seed=n)
         dB = mm.DeviceArray.from_numpy(B16)
-        chunks = (n + 255) // 256
         d8 = mm.DeviceArray((K * n,), np.uint8)
-        ds = mm.DeviceArray((chunks,), np.uint8)
-        work = mm.DeviceArray((K // 16 * chunks,), np.float32)
+        ds = mm.DeviceArray((lib.bcsr_b8_scale_bytes(K, n),), np.uint8)
         C1 = mm.DeviceArray((M, n), np.float32)
         C2 = mm.DeviceArray((M, n), np.float32)
         C1.fill_bytes(0xFF)
-        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, n, d8.ptr, ds.ptr, work.ptr, None))
+        mm.check(lib.bcsr_quantize_b_e4m3(dB.ptr, K, n, d8.ptr, ds.ptr, None, None))
         mm.check(lib.bcsr_spmm_b8(plan._handle, d8.ptr, ds.ptr, C1.ptr, n, None, None))
         plan.spmm_ptr(dB.ptr, C2.ptr, n)
         mm.synchronize()

@@ -7,7 +7,7 @@ import sys
 
 import pytest
 
-from matrixmultiplicationcuda_amd import batch_bench, collect_timings, scale_table, train_costmodel
+from matrixmultiplicationcuda_amd import batch_bench, collect_timings, learned, scale_table, train_costmodel
 
 pytestmark = pytest.mark.gpu
 
@@ -58,5 +58,7 @@ def test_collect_timings_then_train(tmp_path, capsys):
     assert train_costmodel.main(["--db", str(db), "--out", str(model), "--split", "424244",
                                  "--steps", "50"]) == 0
     meta = json.loads([x for x in capsys.readouterr().out.splitlines() if x.startswith("{")][-1])
-    assert meta["train"] == 2 and meta["held_out"] == 1 and "regret" in meta
+    # of the 2 training records one fits the model and one chooses its margin
+    assert meta["train"] == 1 and meta["validation"] == 1 and meta["held_out"] == 1
+    assert "regret" in meta and meta["margin"] in learned.MARGINS
     assert model.exists()

@@ -104,21 +104,33 @@ def _smallest_scale(amax):
     return max(k, -127)
 
 
-def test_b_quantiser_column_chunks():
-    """One scale per 256-column chunk over all rows; the last chunk ragged."""
+def test_b_quantiser_slab_chunks():
+    """One scale per (16-row slab, 256-column chunk), from that patch's own
+    absmax; the last chunk ragged; an all-zero patch gets 2^0."""
     rng = np.random.default_rng(5)
     K, N = 48, 600
     B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
     B[16:32, 256:512] *= 1e-3
     B[32:48, 512:] *= 50.0
+    B[0:16, 512:] = 0.0
     q, sc = oracle.quant_e4m3_B(B)
-    assert sc.shape == (3,)
+    assert sc.shape == (3, 3)
     for c in range(3):
-        blk = B[:, 256 * c:min(256 * c + 256, N)]
-        k = int(sc[c]) - 127
-        assert k == _smallest_scale(float(np.abs(blk).max()))
-        want = _torch_e4m3(blk / np.float32(2.0**k))
-        assert np.array_equal(q[:, 256 * c:min(256 * c + 256, N)], want)
+        for s in range(3):
+            blk = B[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)]
+            amax = float(np.abs(blk).max())
+            k = int(sc[c, s]) - 127
+            assert k == (_smallest_scale(amax) if amax > 0 else 0)
+            want = _torch_e4m3(blk / np.float32(2.0**k))
+            assert np.array_equal(q[16 * s:16 * s + 16, 256 * c:min(256 * c + 256, N)], want)
+    # the scaled patches differ: slab 1 of chunk 1 and slab 2 of chunk 2
+    assert sc[1, 1] < sc[1, 0] and sc[2, 2] > sc[2, 1]
+    # dequantised B: every value within half an E4M3 step of the input
+    Bq = oracle.quantized_B("fp8", B.astype(np.float16).view(np.uint16))
+    B16 = B.astype(np.float16).astype(np.float32)
+    nz = B16 != 0
+    assert np.all(np.abs(Bq - B16)[nz] <= np.abs(B16)[nz] * 2.0**-4 + 2.0**-9 * np.repeat(
+        np.repeat(oracle.e8m0_to_f32(sc).T, 16, axis=0), 256, axis=1)[:K, :N][nz])
 
 
 @pytest.mark.parametrize("bh,bw", [(16, 16), (32, 16), (64, 16), (16, 32)])
