@@ -120,17 +120,19 @@ def test_fp8_b_quantiser_matches_oracle():
     assert lib.bcsr_quantize_b_e4m3(dB.ptr, K, N, d8.ptr, ds.offset(8), None, None) == -3  # misaligned scales
 
 
-@pytest.mark.parametrize("bh,N,tile", [(16, 512, 0), (16, 512, 64), (16, 96, 32), (32, 384, 0), (64, 272, 0),
-                                       (64, 128, 64)])
-def test_fp8_slab_scales_differ(bh, N, tile):
+@pytest.mark.parametrize("bh,N,tile,K", [(16, 512, 0, 1024), (16, 512, 64, 1024), (16, 512, 256, 1024),
+                                         (16, 96, 32, 1024), (32, 384, 0, 1024), (64, 272, 0, 1024),
+                                         (64, 128, 64, 1024), (16, 256, 0, 20480)])
+def test_fp8_slab_scales_differ(bh, N, tile, K):
     """B whose 16-row slabs differ in magnitude (2^-6 .. 2^5) in its first
     256 columns only: there each slab keeps its own E8M0 scale and the
     product takes two MFMAs per octet (even slots, odd slots), the other
-    chunk stays on the one-MFMA path.  Against the exact product of the
+    chunk stays on the one-MFMA path; K = 20480 gives 1280 slabs, more than
+    one read of the chunk's scales.  Against the exact product of the
     stored values (2^-11, as every FP8 product) and, per entry, within the
     uniform-B bounds of the FP64 product (the slab scales follow the
     magnitudes, so the relative error stays that of uniform B)."""
-    M, K = 512, 1024
+    M = 512 if K <= 1024 else 128
     h32, data32, B16 = case(M, K, N, bh, dist="uniform")
     rng = np.random.default_rng(bh * 1000 + N)
     mags = (2.0 ** rng.integers(-6, 6, K // 16)).astype(np.float32)
@@ -140,7 +142,7 @@ def test_fp8_slab_scales_differ(bh, N, tile):
     _, sc = oracle.quant_e4m3_B(B16.astype(np.float32))
     assert len(set(sc[0].tolist())) > 4
     assert N <= 256 or len(set(sc[1].tolist())) == 1
-    log(f"bh={bh} N={N} tile={tile} nnzb={h32.nnzb}: chunk 0 slab scales {sorted(set(sc[0].tolist()))}")
+    log(f"bh={bh} N={N} tile={tile} K={K} nnzb={h32.nnzb}: chunk 0 slab scales {sorted(set(sc[0].tolist()))}")
     C = run_mode(h32, B16, "fp8", mm.make_cfg(tile_n=tile) if tile else None).astype(np.float64)
     Aq = np.asarray(oracle.dequantized_blocks("fp8", data32, bh, 16, h32.hdr, h32.idx), dtype=np.float32)
     Bq = oracle.quantized_B("fp8", B16.view(np.uint16))

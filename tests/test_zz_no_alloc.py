@@ -44,15 +44,21 @@ def run_both(plan, B16, M, N, cfg=None):
     return outs
 
 
-@pytest.mark.parametrize("precision", ["fp8", "f16"])
-def test_noalloc_equals_spmm_on_two_calls(precision):
+@pytest.mark.parametrize("precision,slab_mags", [("fp8", False), ("fp8", True), ("f16", False)])
+def test_noalloc_equals_spmm_on_two_calls(precision, slab_mags):
+    """slab_mags: B's 16-row slabs scaled by 2^-6 .. 2^5, so FP8 chunks carry
+    different slab scales (the two-MFMA path)."""
     assert mm.device_count() > 0
     mm.check(mm.lib().bcsr_set_device(0))
     # the second call is smaller (fewer rows, K and N), so stale scratch
     # contents of the first would show
     for (M, K, N, seed) in [(1024, 4096, 512, 11), (512, 1024, 96, 12)]:
-        log(f"noalloc {precision}: M={M} K={K} N={N}")
+        log(f"noalloc {precision} slab magnitudes {slab_mags}: M={M} K={K} N={N}")
         h, B16 = product(M, K, N, precision, seed)
+        if slab_mags:
+            rng = np.random.default_rng(seed)
+            B16 = (B16.astype(np.float32) * np.repeat(2.0 ** rng.integers(-6, 6, K // 16), 16)[:, None]).astype(
+                np.float16)
         plan = mm.Plan(h, precision)
         want, got = run_both(plan, B16, M, N)
         assert np.isfinite(got).all()
