@@ -5,7 +5,9 @@ log-uniform on [1e-3, 1e2] with random sign; B = U(-1, 1) fp16, seed 456).
 Schemes (A storage / B operand):
   shipped        E8M0 scale per 16 x 32 tile of A (two device blocks, the
                  scale the CDNA4 scaled MFMA applies) / FP8 B with one E8M0
-                 scale per 256-column chunk over all K (the library's mode)
+                 scale per 16-row slab and 256-column chunk (the library's
+                 mode; for this U(-1, 1) B every slab of a chunk gets the
+                 same scale)
   per_block_f32  FP32 absmax scale per 16 x 16 block of A / the same FP8 B
   a_only         the shipped A / B kept fp16
   a_only_f32     the per-block FP32 A / B kept fp16
