@@ -130,7 +130,8 @@ int64_t bcsr_b8_scale_bytes(int32_t K, int32_t N);
  * multiplies one B by several FP8 plans, or the same B more than once,
  * quantises it once.  FP8 plans only (BCSR_ERR_UNSUPPORTED otherwise); B8
  * and C 16-byte aligned, N % 16 == 0; cfg may be NULL.  C is identical to
- * bcsr_spmm's on the B the quantiser was given.  No reference counterpart. */
+ * bcsr_spmm's on the B the quantiser was given.  The scales 16-byte aligned
+ * (BCSR_ERR_SHAPE otherwise).  No reference counterpart. */
 int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, float *C, int32_t N,
                  const bcsr_launch_cfg *cfg, void *stream);
 /* bcsr_spmm_cfg that allocates nothing.  No counterpart in the reference: it

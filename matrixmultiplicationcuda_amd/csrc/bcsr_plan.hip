@@ -665,6 +665,7 @@ int bcsr_spmm_b8(const bcsr_plan *plan, const void *B8, const void *scales, floa
     if (plan->info.precision != BCSR_PREC_FP8_E4M3) return BCSR_ERR_UNSUPPORTED;
     if (N == 0 || plan->info.rows == 0) return BCSR_OK;
     if (!scales) return BCSR_ERR_INVALID_ARG;
+    if (reinterpret_cast<uintptr_t>(scales) & 15u) return BCSR_ERR_SHAPE;  // read 16 bytes at a time
     LaunchShape ls;
     rc = resolve_launch(plan, N, cfg, ls);
     if (rc != BCSR_OK) return rc;

@@ -184,6 +184,7 @@ def test_spmm_on_prequantised_b_equals_spmm(bh, N):
         mm.synchronize()
         assert np.array_equal(C1.numpy(), C2.numpy()), (bh, n)
     assert lib.bcsr_spmm_b8(plan._handle, d8.offset(8), ds.ptr, C1.ptr, 48, None, None) == -3
+    assert lib.bcsr_spmm_b8(plan._handle, d8.ptr, ds.offset(4), C1.ptr, 48, None, None) == -3
     p16 = mm.Plan(h, "f16")
     assert lib.bcsr_spmm_b8(p16._handle, d8.ptr, ds.ptr, C1.ptr, 48, None, None) == -7
     p16.close()
