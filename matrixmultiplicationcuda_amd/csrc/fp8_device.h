@@ -33,6 +33,19 @@ constexpr int kChunk = 256;  // columns per B scale
 // Bytes from one chunk's slab scales to the next (16-byte aligned rows).
 __host__ __device__ constexpr int scale_stride(int slabs) { return (slabs + 15) & ~15; }
 
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// p[i] read through the constant address space, so that a wave-uniform
+// address always becomes a scalar load.  The plan's row pointers, octet
+// columns and A scales are written before any launch and never by a kernel;
+// read as generic pointers, the compiler proves that only while the kernel
+// holds no store it cannot place, and otherwise turns these reads into vector
+// loads that wait for every load in flight (a vmcnt(0) in the octet loop).
+template <typename T>
+__device__ __forceinline__ T const_load(const T *p, size_t i) {
+    return reinterpret_cast<const __attribute__((address_space(4))) T *>(reinterpret_cast<uintptr_t>(p))[i];
+}
+
 // Smallest power of two 2^k (k in [-127, 127]) with absmax / 2^k <= 448, as
 // the E8M0 byte k + 127; 127 for an all-zero group.
 __device__ __forceinline__ unsigned e8m0_for_absmax(float amax) {
@@ -202,10 +215,10 @@ __device__ __forceinline__ void fp8_work_mixed(const SpmmArgs &a, int br, int ti
     for (int s = 0; s < RS; ++s)
 #pragma unroll
         for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
+    const int q0 = const_load(a.grp_ptr, br), q1 = const_load(a.grp_ptr, br + 1);
     for (int q = q0; q < q1; ++q) {
-        const int4 c03 = reinterpret_cast<const int4 *>(a.grp_col)[2 * q];
-        const int4 c47 = reinterpret_cast<const int4 *>(a.grp_col)[2 * q + 1];
+        const i32x4 c03 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * q);
+        const i32x4 c47 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * q + 1);
         const int cs[8] = {c03.x, c03.y, c03.z, c03.w, c47.x, c47.y, c47.z, c47.w};
         // this lane's data slots g, g + 4; its scale group g = slots 2 g, 2 g + 1
         int cl = cs[0], ch = cs[4], ce = cs[0], co = cs[1];
@@ -224,7 +237,7 @@ __device__ __forceinline__ void fp8_work_mixed(const SpmmArgs &a, int br, int ti
             const unsigned aoff = static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32);
             av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
             av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
-            sa[s] = static_cast<int>((a.ascale[static_cast<size_t>(q) * RS + s] >> (8 * g)) & 0xffu);
+            sa[s] = static_cast<int>((const_load(a.ascale, static_cast<size_t>(q) * RS + s) >> (8 * g)) & 0xffu);
         }
         const unsigned lo = static_cast<unsigned>(cl) * (16u * N) + lane_off;
         const unsigned hi = static_cast<unsigned>(ch) * (16u * N) + lane_off;
@@ -290,7 +303,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
         for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int q0 = a.grp_ptr[br], q1 = a.grp_ptr[br + 1];
+    const int q0 = const_load(a.grp_ptr, br), q1 = const_load(a.grp_ptr, br + 1);
     if (q0 < q1) {
         // B's scales exist only where B has rows (a row with octets implies
         // K > 0; with K = 0 no quantised B is made); sb, the chunk's one
@@ -305,8 +318,8 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         // last octet) and are picked per lane group by selects, not branches
         auto bases = [&](int q, unsigned &lo, unsigned &hi) {
             const int qc = q < q1 ? q : q1 - 1;
-            const int4 c03 = reinterpret_cast<const int4 *>(a.grp_col)[2 * qc];
-            const int4 c47 = reinterpret_cast<const int4 *>(a.grp_col)[2 * qc + 1];
+            const i32x4 c03 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * qc);
+            const i32x4 c47 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * qc + 1);
             int cl = c03.x, ch = c47.x;
             cl = g == 1 ? c03.y : cl;
             ch = g == 1 ? c47.y : ch;
@@ -336,7 +349,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                 av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
                 av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
                 const int qc = q < q1 ? q : q1 - 1;
-                const unsigned sc = a.ascale[static_cast<size_t>(qc) * RS + s];
+                const unsigned sc = const_load(a.ascale, static_cast<size_t>(qc) * RS + s);
                 sa[s] = static_cast<int>((sc >> (8 * g)) & 0xffu);
             }
         };
@@ -368,6 +381,20 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                     acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                         __builtin_bit_cast(i32x8, av[s]), bv, acc[s][e], 0, 0, 0, sa[s], 0, sb);
             }
+            // The empty statements use each result where it is made.  Without
+            // them the compiler sinks these MFMAs into the blocks that follow
+            // the next loads, past the schedule barriers: the buffer just read
+            // then lives on beside the next one, and an accumulator whose old
+            // value is still live gets a second register set (the MFMA form
+            // whose result may not overlap its input) -- over 210 VGPRs and 2
+            // waves per SIMD for one row set at 128 columns, against 141 and 3.
+#pragma unroll
+            for (int k = 0; k < HB; ++k) {
+                const int e = half * HB + k;
+                if (e >= EPL) continue;
+#pragma unroll
+                for (int s = 0; s < RS; ++s) asm("" : "+v"(acc[s][e]));
+            }
         };
         auto advance = [&](int q) {
             lo_c = lo_n;
@@ -385,21 +412,18 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             return;
         }
         if constexpr (RS == 1 && EPL >= 4) {
-            // Each half of the loop body checks, before loading the next octet,
-            // whether the row has one; the row's last octet is finished in the
-            // branch that leaves the loop, so nothing is loaded past the row's
-            // end (such loads would read zeros but still occupy the load path)
-            // and the main path through the loop has no merge of branches.
-            for (int q = q0;; q += 2) {
+            // The loop runs while the row has two more octets after the
+            // current one, so every load it issues is inside the row and it
+            // has one exit; the last one or two octets follow it.  Nothing is
+            // loaded past the row's end (such loads would read zeros but still
+            // occupy the load path).
+            int q = q0;
+            for (; q + 2 < q1; q += 2) {
                 // octet q (A in avA)
                 issue_half(lo_c, hi_c, 1, bufY);
                 __builtin_amdgcn_sched_barrier(0);
                 compute_half(0, bufX, avA, saA);
                 __builtin_amdgcn_sched_barrier(0);
-                if (q + 1 >= q1) {
-                    compute_half(1, bufY, avA, saA);
-                    break;
-                }
                 load_a(q + 1, avB, saB);
                 issue_half(lo_n, hi_n, 0, bufX);
                 __builtin_amdgcn_sched_barrier(0);
@@ -411,16 +435,31 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                 __builtin_amdgcn_sched_barrier(0);
                 compute_half(0, bufX, avB, saB);
                 __builtin_amdgcn_sched_barrier(0);
-                if (q + 2 >= q1) {
-                    compute_half(1, bufY, avB, saB);
-                    break;
-                }
                 load_a(q + 2, avA, saA);
                 issue_half(lo_n, hi_n, 0, bufX);
                 __builtin_amdgcn_sched_barrier(0);
                 compute_half(1, bufY, avB, saB);
                 __builtin_amdgcn_sched_barrier(0);
                 advance(q + 3);
+            }
+            // octet q (A in avA, first half of B in bufX), then octet q + 1
+            // if the row has it
+            issue_half(lo_c, hi_c, 1, bufY);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_half(0, bufX, avA, saA);
+            __builtin_amdgcn_sched_barrier(0);
+            if (q + 1 < q1) {
+                load_a(q + 1, avB, saB);
+                issue_half(lo_n, hi_n, 0, bufX);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(1, bufY, avA, saA);
+                __builtin_amdgcn_sched_barrier(0);
+                issue_half(lo_n, hi_n, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_half(0, bufX, avB, saB);
+                compute_half(1, bufY, avB, saB);
+            } else {
+                compute_half(1, bufY, avA, saA);
             }
         } else {
             // (with more row sets or narrow tiles the early exits below cost
