@@ -140,6 +140,10 @@ int launch_fp8(bcsr_plan *plan, const LaunchShape &ls, const void *B, float *C, 
 int launch_fp8_product(const bcsr_plan *plan, const LaunchShape &ls, const uint8_t *B8, const uint8_t *bscale,
                        float *C, int N, hipStream_t stream);
 bool pair_shape_supported(int precision, int rs, int epl, int variant);
+// FP8 product shapes (row sets x 16-column blocks per tile) with a kernel:
+// all but 64-row blocks at 256 columns, whose 256 accumulator registers leave
+// no room for the operands (the compiler spilled it, and now fails on it)
+constexpr bool fp8_shape_supported(int rs, int epl) { return !(rs == 4 && epl == 16); }
 // The slab-streaming panel kernel (variant 3).
 bool panel_supported(const bcsr_plan *plan);
 int panel_grid(const bcsr_plan *plan, int n_tiles, int map_mode);

@@ -479,7 +479,7 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         ls.variant = 0;
         if (!pair_shape_supported(plan->info.precision, plan->rs, tile_n / 16, 0)) return BCSR_ERR_UNSUPPORTED;
     }
-    if (fp8 && ls.variant == 2) return BCSR_ERR_UNSUPPORTED;
+    if (fp8 && (ls.variant == 2 || !fp8_shape_supported(plan->rs, tile_n / 16))) return BCSR_ERR_UNSUPPORTED;
     // buffer offsets are 32-bit: larger B falls back to 64-bit global loads
     if (!fp8 && static_cast<int64_t>(plan->info.cols) * N * 2 > INT32_MAX) ls.variant = 2;
     if (fp8 && (static_cast<int64_t>(plan->info.cols) * N > INT32_MAX ||

@@ -193,90 +193,6 @@ __device__ __forceinline__ int scale_row_uniform(const SpmmArgs &a, int h, u32x4
     return __builtin_amdgcn_ballot_w64(differs) ? -1 : s0;
 }
 
-// Work item (br, tile) for a chunk whose slabs have different scales: octet by
-// octet (no pipelining: this is the path of uncommon B), two MFMAs per 16
-// columns -- even slots with the even slabs' scales, odd slots with the odd
-// slabs' -- so every slab keeps its own scale.  Lane group g provides the B
-// scale of scale group g (slots 2 g, 2 g + 1) and holds the data of slots g
-// and g + 4: zeroing A in the odd (even) lane groups leaves only the even
-// (odd) slots in the sum.
-template <int RS, int EPL>
-__device__ __forceinline__ void fp8_work_mixed(const SpmmArgs &a, int br, int tile) {
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, j = lane & 15;
-    const int n_base = tile * (16 * EPL);
-    const unsigned N = static_cast<unsigned>(a.N);
-    const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(a.B8, a.b_bytes);
-    const __amdgpu_buffer_rsrc_t rsrc_a = make_rsrc(a.vals, a.a_bytes);
-    const unsigned lane_off = static_cast<unsigned>(n_base + j) * 16u;
-    const uint8_t *sc_chunk = a.b8scale + static_cast<size_t>(n_base / kChunk) * scale_stride(a.b_slabs);
-    f32x4 acc[RS][EPL];
-#pragma unroll
-    for (int s = 0; s < RS; ++s)
-#pragma unroll
-        for (int e = 0; e < EPL; ++e) acc[s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int q0 = const_load(a.grp_ptr, br), q1 = const_load(a.grp_ptr, br + 1);
-    for (int q = q0; q < q1; ++q) {
-        const i32x4 c03 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * q);
-        const i32x4 c47 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * q + 1);
-        const int cs[8] = {c03.x, c03.y, c03.z, c03.w, c47.x, c47.y, c47.z, c47.w};
-        // this lane's data slots g, g + 4; its scale group g = slots 2 g, 2 g + 1
-        int cl = cs[0], ch = cs[4], ce = cs[0], co = cs[1];
-#pragma unroll
-        for (int t = 1; t < 4; ++t) {
-            cl = g == t ? cs[t] : cl;
-            ch = g == t ? cs[t + 4] : ch;
-            ce = g == t ? cs[2 * t] : ce;
-            co = g == t ? cs[2 * t + 1] : co;
-        }
-        const int sbe = sc_chunk[ce], sbo = sc_chunk[co];
-        u32x8 av[RS];
-        int sa[RS];
-#pragma unroll
-        for (int s = 0; s < RS; ++s) {
-            const unsigned aoff = static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32);
-            av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
-            av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
-            sa[s] = static_cast<int>((const_load(a.ascale, static_cast<size_t>(q) * RS + s) >> (8 * g)) & 0xffu);
-        }
-        const unsigned lo = static_cast<unsigned>(cl) * (16u * N) + lane_off;
-        const unsigned hi = static_cast<unsigned>(ch) * (16u * N) + lane_off;
-#pragma unroll
-        for (int e = 0; e < EPL; ++e) {
-            const bool ok = n_base + 16 * e + j < a.N;
-            u32x8 b;
-            b.lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ok ? lo + 256u * e : kOutOfRange), 0, 0);
-            b.hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ok ? hi + 256u * e : kOutOfRange), 0, 0);
-            const i32x8 bv = __builtin_bit_cast(i32x8, b);
-#pragma unroll
-            for (int s = 0; s < RS; ++s) {
-#pragma unroll
-                for (int odd = 0; odd < 2; ++odd) {
-                    // A of this pass's lane groups, zeros in the others
-                    const bool keep = (g & 1) == odd;
-                    i32x8 am;
-#pragma unroll
-                    for (int d = 0; d < 8; ++d) am[d] = keep ? static_cast<int>(av[s][d]) : 0;
-                    acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(am, bv, acc[s][e], 0, 0, 0, sa[s],
-                                                                                 0, odd ? sbo : sbe);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) {
-        const int n = n_base + 16 * e + j;
-        if (n >= a.N) continue;
-#pragma unroll
-        for (int s = 0; s < RS; ++s)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const size_t row = static_cast<size_t>(br) * (16 * RS) + 16 * s + 4 * g + r;
-                a.C[row * static_cast<size_t>(a.N) + n] = acc[s][e][r];
-            }
-    }
-}
-
 // Work item (device block-row br, tile of 16 * EPL columns).  MFMA e covers
 // columns tile + 16 e + j for lane j of each lane group.  The B loads of the
 // row's octets are issued half an octet ahead of the MFMAs that use them
@@ -353,6 +269,26 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                 sa[s] = static_cast<int>((sc >> (8 * g)) & 0xffu);
             }
         };
+        // For a chunk whose slab scales differ: the B scales that lane group
+        // g provides for scale group g of octet q, those of the slabs of its
+        // slots 2 g (sbe) and 2 g + 1 (sbo), read from the chunk's scale row
+        // with the octet's A (an octet past the row's end reads the last one's)
+        const __amdgpu_buffer_rsrc_t rsrc_s = scale_row_rsrc(a, n_base / kChunk);
+        auto load_bscale = [&](int q, int &sbe, int &sbo) {
+            const int qc = q < q1 ? q : q1 - 1;
+            const i32x4 c03 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * qc);
+            const i32x4 c47 = const_load(reinterpret_cast<const i32x4 *>(a.grp_col), 2 * qc + 1);
+            const int ce = g < 2 ? (g == 0 ? c03.x : c03.z) : (g == 2 ? c47.x : c47.z);
+            const int co = g < 2 ? (g == 0 ? c03.y : c03.w) : (g == 2 ? c47.y : c47.w);
+            sbe = __builtin_amdgcn_raw_buffer_load_b8(rsrc_s, ce, 0, 0);
+            sbo = __builtin_amdgcn_raw_buffer_load_b8(rsrc_s, co, 0, 0);
+        };
+        // Lanes of the even (odd) lane groups hold the octet's even (odd)
+        // slots.  mixed: the chunk's slab scales differ (set after the first
+        // loads); keep_first masks A for the first MFMA of each 16 columns.
+        const unsigned keep_even = (g & 1) ? 0u : ~0u;
+        bool mixed = false;
+        unsigned keep_first = ~0u;
 
         // B for half an octet (EPL / 2 column blocks) per step; steps
         // alternate between two buffers: the loads of step u + 1 are issued
@@ -363,6 +299,8 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         u32x8 bufX[HB], bufY[HB];
         u32x8 avA[RS], avB[RS];
         int saA[RS], saB[RS];
+        // B scales of the first and second MFMA, per octet (A's buffers)
+        int sbeA = 0, sboA = 0, sbeB = 0, sboB = 0;
         unsigned lo_c, hi_c, lo_n, hi_n;
         bases(q0, lo_c, hi_c);
         bases(q0 + 1, lo_n, hi_n);
@@ -370,7 +308,17 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
             for (int k = 0; k < HB; ++k) load_b(lo, hi, half * HB + k, buf[k]);
         };
-        auto compute_half = [&](int half, const u32x8 (&buf)[HB], const u32x8 (&av)[RS], const int (&sa)[RS]) {
+        // One MFMA per 16 columns with the chunk's one B scale; for a chunk
+        // whose slab scales differ two, one over the even slots (A's odd lane
+        // groups zeroed) with the even slabs' scales and one over the odd
+        // slots with theirs, so every slab keeps its own scale.
+        // The first MFMA of each 16 columns runs always: for a uniform chunk
+        // over all slots with the chunk's scale, else over the even slots (A's
+        // odd lane groups zeroed) with the even slabs' scales; for a chunk
+        // whose slab scales differ a second one adds the odd slots with
+        // theirs, so every slab keeps its own scale.  mixed is wave-uniform.
+        auto compute_half = [&](int half, const u32x8 (&buf)[HB], const u32x8 (&av)[RS], const int (&sa)[RS],
+                                int sbe, int sbo) {
 #pragma unroll
             for (int k = 0; k < HB; ++k) {
                 const int e = half * HB + k;
@@ -379,7 +327,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
                 for (int s = 0; s < RS; ++s)
                     acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        __builtin_bit_cast(i32x8, av[s]), bv, acc[s][e], 0, 0, 0, sa[s], 0, sb);
+                        __builtin_bit_cast(i32x8, av[s] & keep_first), bv, acc[s][e], 0, 0, 0, sa[s], 0, sbe);
             }
             // The empty statements use each result where it is made.  Without
             // them the compiler sinks these MFMAs into the blocks that follow
@@ -387,7 +335,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             // then lives on beside the next one, and an accumulator whose old
             // value is still live gets a second register set (the MFMA form
             // whose result may not overlap its input) -- over 210 VGPRs and 2
-            // waves per SIMD for one row set at 128 columns, against 141 and 3.
+            // waves per SIMD for one row set at 128 columns, against 155 and 3.
 #pragma unroll
             for (int k = 0; k < HB; ++k) {
                 const int e = half * HB + k;
@@ -395,100 +343,135 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
                 for (int s = 0; s < RS; ++s) asm("" : "+v"(acc[s][e]));
             }
+            if (mixed) {
+#pragma unroll
+                for (int k = 0; k < HB; ++k) {
+                    const int e = half * HB + k;
+                    if (e >= EPL) continue;
+                    const i32x8 bv = __builtin_bit_cast(i32x8, buf[k]);
+#pragma unroll
+                    for (int s = 0; s < RS; ++s)
+                        acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            __builtin_bit_cast(i32x8, av[s] & ~keep_first), bv, acc[s][e], 0, 0, 0, sa[s], 0, sbo);
+                }
+#pragma unroll
+                for (int k = 0; k < HB; ++k) {
+                    const int e = half * HB + k;
+                    if (e >= EPL) continue;
+#pragma unroll
+                    for (int s = 0; s < RS; ++s) asm("" : "+v"(acc[s][e]));
+                }
+            }
         };
         auto advance = [&](int q) {
             lo_c = lo_n;
             hi_c = hi_n;
             bases(q, lo_n, hi_n);
         };
+        // the octet loop (for a chunk whose slab scales differ each octet's A
+        // also brings its B scales)
+        auto octets = [&]() {
+            auto next_a = [&](int q, u32x8 (&av)[RS], int (&sa)[RS], int &sbe, int &sbo) {
+                load_a(q, av, sa);
+                if (mixed) load_bscale(q, sbe, sbo);
+            };
+            auto half_x = [&](int half, const u32x8 (&buf)[HB], bool second) {
+                if (second)
+                    compute_half(half, buf, avB, saB, sbeB, sboB);
+                else
+                    compute_half(half, buf, avA, saA, sbeA, sboA);
+            };
+            if constexpr (RS == 1 && EPL >= 4) {
+                // The loop runs while the row has two more octets after the
+                // current one, so every load it issues is inside the row and it
+                // has one exit; the last one or two octets follow it.  Nothing is
+                // loaded past the row's end (such loads would read zeros but still
+                // occupy the load path).
+                int q = q0;
+                for (; q + 2 < q1; q += 2) {
+                    // octet q (A in avA)
+                    issue_half(lo_c, hi_c, 1, bufY);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(0, bufX, false);
+                    __builtin_amdgcn_sched_barrier(0);
+                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    issue_half(lo_n, hi_n, 0, bufX);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(1, bufY, false);
+                    __builtin_amdgcn_sched_barrier(0);
+                    advance(q + 2);
+                    // octet q + 1 (A in avB)
+                    issue_half(lo_c, hi_c, 1, bufY);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(0, bufX, true);
+                    __builtin_amdgcn_sched_barrier(0);
+                    next_a(q + 2, avA, saA, sbeA, sboA);
+                    issue_half(lo_n, hi_n, 0, bufX);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(1, bufY, true);
+                    __builtin_amdgcn_sched_barrier(0);
+                    advance(q + 3);
+                }
+                // octet q (A in avA, first half of B in bufX), then octet q + 1
+                // if the row has it
+                issue_half(lo_c, hi_c, 1, bufY);
+                __builtin_amdgcn_sched_barrier(0);
+                half_x(0, bufX, false);
+                __builtin_amdgcn_sched_barrier(0);
+                if (q + 1 < q1) {
+                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    issue_half(lo_n, hi_n, 0, bufX);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(1, bufY, false);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue_half(lo_n, hi_n, 1, bufY);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(0, bufX, true);
+                    half_x(1, bufY, true);
+                } else {
+                    half_x(1, bufY, false);
+                }
+            } else {
+                // (with more row sets or narrow tiles the early exits below cost
+                // registers, and so occupancy: the loads past the row's end stay)
+                for (int q = q0; q < q1; q += 2) {
+                    // octet q (A in avA)
+                    issue_half(lo_c, hi_c, 1, bufY);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(0, bufX, false);
+                    __builtin_amdgcn_sched_barrier(0);
+                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    issue_half(lo_n, hi_n, 0, bufX);
+                    __builtin_amdgcn_sched_barrier(0);
+                    half_x(1, bufY, false);
+                    __builtin_amdgcn_sched_barrier(0);
+                    advance(q + 2);
+                    // octet q + 1 (A in avB); past the row's end its loads read zeros
+                    issue_half(lo_c, hi_c, 1, bufY);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (q + 1 < q1) half_x(0, bufX, true);
+                    __builtin_amdgcn_sched_barrier(0);
+                    next_a(q + 2, avA, saA, sbeA, sboA);
+                    issue_half(lo_n, hi_n, 0, bufX);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (q + 1 < q1) half_x(1, bufY, true);
+                    __builtin_amdgcn_sched_barrier(0);
+                    advance(q + 3);
+                }
+            }
+        };
         load_a(q0, avA, saA);
         issue_half(lo_c, hi_c, 0, bufX);
-        // checked after the pipeline's first loads are issued; a chunk whose
-        // slab scales differ takes the two-MFMA path (the loads issued here
-        // are then unused)
+        // checked after the pipeline's first loads are issued
         sb = scale_row_uniform(a, n_base / kChunk, srow);
-        if (sb < 0) {
-            fp8_work_mixed<RS, EPL>(a, br, tile);
-            return;
-        }
-        if constexpr (RS == 1 && EPL >= 4) {
-            // The loop runs while the row has two more octets after the
-            // current one, so every load it issues is inside the row and it
-            // has one exit; the last one or two octets follow it.  Nothing is
-            // loaded past the row's end (such loads would read zeros but still
-            // occupy the load path).
-            int q = q0;
-            for (; q + 2 < q1; q += 2) {
-                // octet q (A in avA)
-                issue_half(lo_c, hi_c, 1, bufY);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(0, bufX, avA, saA);
-                __builtin_amdgcn_sched_barrier(0);
-                load_a(q + 1, avB, saB);
-                issue_half(lo_n, hi_n, 0, bufX);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(1, bufY, avA, saA);
-                __builtin_amdgcn_sched_barrier(0);
-                advance(q + 2);
-                // octet q + 1 (A in avB)
-                issue_half(lo_c, hi_c, 1, bufY);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(0, bufX, avB, saB);
-                __builtin_amdgcn_sched_barrier(0);
-                load_a(q + 2, avA, saA);
-                issue_half(lo_n, hi_n, 0, bufX);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(1, bufY, avB, saB);
-                __builtin_amdgcn_sched_barrier(0);
-                advance(q + 3);
-            }
-            // octet q (A in avA, first half of B in bufX), then octet q + 1
-            // if the row has it
-            issue_half(lo_c, hi_c, 1, bufY);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_half(0, bufX, avA, saA);
-            __builtin_amdgcn_sched_barrier(0);
-            if (q + 1 < q1) {
-                load_a(q + 1, avB, saB);
-                issue_half(lo_n, hi_n, 0, bufX);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(1, bufY, avA, saA);
-                __builtin_amdgcn_sched_barrier(0);
-                issue_half(lo_n, hi_n, 1, bufY);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(0, bufX, avB, saB);
-                compute_half(1, bufY, avB, saB);
-            } else {
-                compute_half(1, bufY, avA, saA);
-            }
+        mixed = sb < 0;
+        keep_first = mixed ? keep_even : ~0u;
+        if (mixed) {
+            load_bscale(q0, sbeA, sboA);
         } else {
-            // (with more row sets or narrow tiles the early exits below cost
-            // registers, and so occupancy: the loads past the row's end stay)
-            for (int q = q0; q < q1; q += 2) {
-                // octet q (A in avA)
-                issue_half(lo_c, hi_c, 1, bufY);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(0, bufX, avA, saA);
-                __builtin_amdgcn_sched_barrier(0);
-                load_a(q + 1, avB, saB);
-                issue_half(lo_n, hi_n, 0, bufX);
-                __builtin_amdgcn_sched_barrier(0);
-                compute_half(1, bufY, avA, saA);
-                __builtin_amdgcn_sched_barrier(0);
-                advance(q + 2);
-                // octet q + 1 (A in avB); past the row's end its loads read zeros
-                issue_half(lo_c, hi_c, 1, bufY);
-                __builtin_amdgcn_sched_barrier(0);
-                if (q + 1 < q1) compute_half(0, bufX, avB, saB);
-                __builtin_amdgcn_sched_barrier(0);
-                load_a(q + 2, avA, saA);
-                issue_half(lo_n, hi_n, 0, bufX);
-                __builtin_amdgcn_sched_barrier(0);
-                if (q + 1 < q1) compute_half(1, bufY, avB, saB);
-                __builtin_amdgcn_sched_barrier(0);
-                advance(q + 3);
-            }
+            sbeA = sbeB = sb;
         }
+        octets();
     }
 
 #pragma unroll

@@ -49,7 +49,9 @@ Kernel pick_epl(int epl) {
         case 2: return bcsr_spmm_fp8<RS, 2>;
         case 4: return bcsr_spmm_fp8<RS, 4>;
         case 8: return bcsr_spmm_fp8<RS, 8>;
-        case 16: return bcsr_spmm_fp8<RS, 16>;
+        case 16:
+            if constexpr (fp8_shape_supported(RS, 16)) return bcsr_spmm_fp8<RS, 16>;
+            return nullptr;
         default: return nullptr;
     }
 }

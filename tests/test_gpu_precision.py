@@ -158,6 +158,25 @@ def test_fp8_slab_scales_differ(bh, N, tile, K):
     assert np.all(C[~nz] == 0)
 
 
+def test_fp8_64_row_blocks_have_no_256_column_kernel():
+    """FP8 builds no kernel for 64-row blocks at 256-column tiles (its 256
+    accumulator registers leave no room for the operands): the plan reports
+    that configuration unsupported and a call with it fails with
+    BCSR_ERR_UNSUPPORTED, while 128-column tiles run and match the automatic
+    choice."""
+    h32, _, B16 = case(256, 1024, 512, 64, dist="uniform")
+    plan = mm.Plan(h32, "fp8")
+    try:
+        assert not plan.supports(512, mm.make_cfg(tile_n=256))
+        assert plan.supports(512, mm.make_cfg(tile_n=128))
+        with pytest.raises(mm.BcsrError, match="BCSR_ERR_UNSUPPORTED"):
+            plan.matmul(B16, cfg=mm.make_cfg(tile_n=256))
+        C128 = plan.matmul(B16, cfg=mm.make_cfg(tile_n=128))
+        assert np.array_equal(C128, plan.matmul(B16))
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("bh,N", [(16, 512), (32, 96), (64, 272)])
 def test_spmm_on_prequantised_b_equals_spmm(bh, N):
     """bcsr_spmm_b8 on bcsr_quantize_b_e4m3's output == bcsr_spmm (which
