@@ -177,11 +177,13 @@ def test_fp8_64_row_blocks_have_no_256_column_kernel():
         plan.close()
 
 
+@pytest.mark.parametrize("slab_mags", [False, True])
 @pytest.mark.parametrize("bh,N", [(16, 512), (32, 96), (64, 272)])
-def test_spmm_on_prequantised_b_equals_spmm(bh, N):
+def test_spmm_on_prequantised_b_equals_spmm(bh, N, slab_mags):
     """bcsr_spmm_b8 on bcsr_quantize_b_e4m3's output == bcsr_spmm (which
-    quantises B itself) bit for bit, over two calls with different N; a
-    16-bit plan and a misaligned B8 are refused."""
+    quantises B itself) bit for bit, over two calls with different N, for B
+    of one magnitude and for B whose 16-row slabs differ in magnitude (two
+    MFMAs per 16 columns); a 16-bit plan and a misaligned B8 are refused."""
     lib = mm.lib()
     M, K = 512, 1024
     A32 = mm.generate(M, K, 0.8, "blockrandom", bh=bh, seed=bh, dtype=np.float32)
@@ -191,6 +193,9 @@ def test_spmm_on_prequantised_b_equals_spmm(bh, N):
     plan = mm.Plan(h32, "fp8")
     for n in (N, 48):
         B16 = mm.generate(K, n, 0.0, "random", seed=n)
+        if slab_mags:
+            mags = 2.0 ** np.random.default_rng(n).integers(-6, 6, K // 16)
+            B16 = (B16.astype(np.float32) * np.repeat(mags, 16)[:, None]).astype(np.float16)
         dB = mm.DeviceArray.from_numpy(B16)
         d8 = mm.DeviceArray((K * n,), np.uint8)
         ds = mm.DeviceArray((lib.bcsr_b8_scale_bytes(K, n),), np.uint8)
