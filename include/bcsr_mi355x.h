@@ -50,7 +50,8 @@ enum bcsr_precision {
 typedef struct bcsr_plan bcsr_plan;
 
 typedef struct {
-    int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 */
+    int32_t tile_n;   /* output columns per wave: 0 = auto, else 32, 64, 128 or 256 (FP8 plans
+                       * with 64-row device blocks: up to 128; 256 gives BCSR_ERR_UNSUPPORTED) */
     int32_t xcd_map;  /* -1 = auto, 0 = linear block order, 1 = column tiles pinned to XCDs,
                        * 2 = block-rows pinned to XCDs (all tiles of a row on one XCD),
                        * 3 = tile-major work cut into 8 equal runs, one per XCD (any tile count),
