@@ -308,10 +308,6 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
             for (int k = 0; k < HB; ++k) load_b(lo, hi, half * HB + k, buf[k]);
         };
-        // One MFMA per 16 columns with the chunk's one B scale; for a chunk
-        // whose slab scales differ two, one over the even slots (A's odd lane
-        // groups zeroed) with the even slabs' scales and one over the odd
-        // slots with theirs, so every slab keeps its own scale.
         // The first MFMA of each 16 columns runs always: for a uniform chunk
         // over all slots with the chunk's scale, else over the even slots (A's
         // odd lane groups zeroed) with the even slabs' scales; for a chunk
@@ -368,8 +364,8 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             hi_c = hi_n;
             bases(q, lo_n, hi_n);
         };
-        // the octet loop (for a chunk whose slab scales differ each octet's A
-        // also brings its B scales)
+        // The octet loop.  For a chunk whose slab scales differ, each octet's
+        // A load also brings the octet's B scales.
         auto octets = [&]() {
             auto next_a = [&](int q, u32x8 (&av)[RS], int (&sa)[RS], int &sbe, int &sbo) {
                 load_a(q, av, sa);
