@@ -200,7 +200,13 @@ __device__ __forceinline__ int scale_row_uniform(const SpmmArgs &a, int h, u32x4
 // octet boundaries.  A load past the row's end aims at an out-of-range
 // offset (zeros, no traffic); with one row set and tiles of 64 columns or
 // more the loop leaves before issuing any (see below).
-template <int RS, int EPL>
+//
+// kSplitA chooses how a chunk whose slab scales differ feeds A to its two
+// MFMAs: split by lane group as A loads (one row set; 153 VGPRs at 128
+// columns, and 3 % faster than masking at 256), or masked at the MFMA (more
+// row sets, and the batched kernel, where two more A buffers push its widest
+// shapes past 256 VGPRs into spills).
+template <int RS, int EPL, bool kSplitA = (RS == 1)>
 __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_in) {
     // wave-uniform by construction (scalar index chain)
     const int br = __builtin_amdgcn_readfirstlane(br_in);
@@ -225,6 +231,14 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         // K > 0; with K = 0 no quantised B is made); sb, the chunk's one
         // scale, is read below, after the first loads
         int sb = 0;
+        // the chunk's slab scales differ (set after the first loads); lanes of
+        // the even (odd) lane groups hold the octet's even (odd) slots
+        bool mixed = false;
+        // For such a chunk, A is either split by lane group as it loads
+        // (kSplitA: two loads, zeros from loads aimed out of range) or masked
+        // at the MFMA (keep_first).
+        const unsigned keep_even = (g & 1) ? 0u : ~0u;
+        unsigned keep_first = ~0u;
         // the chunk's slab scales, read before the pipeline's first loads so
         // that waiting for them does not wait for those
         const u32x4 srow = scale_row_load(a, n_base / kChunk);
@@ -257,16 +271,32 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             b.lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(ol), 0, 0);
             b.hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc, static_cast<int>(oh), 0, 0);
         };
+        // A of octet q for the first MFMA of each 16 columns (for a chunk
+        // whose slab scales differ only the even lane groups load, the others
+        // read zeros), and its A scales
         auto load_a = [&](int q, u32x8 (&av)[RS], int (&sa)[RS]) {
 #pragma unroll
             for (int s = 0; s < RS; ++s) {
                 const unsigned aoff =
                     q < q1 ? static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32) : kOutOfRange;
-                av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 0, 0);
-                av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(aoff), 16, 0);
+                const unsigned a1 = kSplitA && mixed && (g & 1) ? kOutOfRange : aoff;
+                av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(a1), 0, 0);
+                av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(a1), 16, 0);
                 const int qc = q < q1 ? q : q1 - 1;
                 const unsigned sc = const_load(a.ascale, static_cast<size_t>(qc) * RS + s);
                 sa[s] = static_cast<int>((sc >> (8 * g)) & 0xffu);
+            }
+        };
+        // for a chunk whose slab scales differ: A of octet q in the odd lane
+        // groups only, for the second MFMA
+        auto load_a_odd = [&](int q, u32x8 (&av)[RS]) {
+#pragma unroll
+            for (int s = 0; s < RS; ++s) {
+                const unsigned aoff =
+                    q < q1 ? static_cast<unsigned>(((static_cast<size_t>(q) * RS + s) * 64 + lane) * 32) : kOutOfRange;
+                const unsigned a2 = (g & 1) ? aoff : kOutOfRange;
+                av[s].lo = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(a2), 0, 0);
+                av[s].hi = __builtin_amdgcn_raw_buffer_load_b128(rsrc_a, static_cast<int>(a2), 16, 0);
             }
         };
         // For a chunk whose slab scales differ: the B scales that lane group
@@ -283,12 +313,6 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             sbe = __builtin_amdgcn_raw_buffer_load_b8(rsrc_s, ce, 0, 0);
             sbo = __builtin_amdgcn_raw_buffer_load_b8(rsrc_s, co, 0, 0);
         };
-        // Lanes of the even (odd) lane groups hold the octet's even (odd)
-        // slots.  mixed: the chunk's slab scales differ (set after the first
-        // loads); keep_first masks A for the first MFMA of each 16 columns.
-        const unsigned keep_even = (g & 1) ? 0u : ~0u;
-        bool mixed = false;
-        unsigned keep_first = ~0u;
 
         // B for half an octet (EPL / 2 column blocks) per step; steps
         // alternate between two buffers: the loads of step u + 1 are issued
@@ -297,7 +321,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         // loaded register is copied (a copy would wait for its load).
         constexpr int HB = EPL / 2 > 0 ? EPL / 2 : 1;
         u32x8 bufX[HB], bufY[HB];
-        u32x8 avA[RS], avB[RS];
+        u32x8 avA[RS], avB[RS], aoA[RS], aoB[RS];
         int saA[RS], saB[RS];
         // B scales of the first and second MFMA, per octet (A's buffers)
         int sbeA = 0, sboA = 0, sbeB = 0, sboB = 0;
@@ -309,12 +333,14 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             for (int k = 0; k < HB; ++k) load_b(lo, hi, half * HB + k, buf[k]);
         };
         // The first MFMA of each 16 columns runs always: for a uniform chunk
-        // over all slots with the chunk's scale, else over the even slots (A's
-        // odd lane groups zeroed) with the even slabs' scales; for a chunk
-        // whose slab scales differ a second one adds the odd slots with
-        // theirs, so every slab keeps its own scale.  mixed is wave-uniform.
-        auto compute_half = [&](int half, const u32x8 (&buf)[HB], const u32x8 (&av)[RS], const int (&sa)[RS],
-                                int sbe, int sbo) {
+        // over all slots with the chunk's scale, else over the even slots (av
+        // holds zeros in the odd lane groups) with the even slabs' scales; for
+        // a chunk whose slab scales differ a second one adds the odd slots (ao)
+        // with theirs, so every slab keeps its own scale.  Zeros come from
+        // loads aimed out of range, so no VALU work splits A.  mixed is
+        // wave-uniform.
+        auto compute_half = [&](int half, const u32x8 (&buf)[HB], const u32x8 (&av)[RS], const u32x8 (&ao)[RS],
+                                const int (&sa)[RS], int sbe, int sbo) {
 #pragma unroll
             for (int k = 0; k < HB; ++k) {
                 const int e = half * HB + k;
@@ -323,7 +349,8 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
                 for (int s = 0; s < RS; ++s)
                     acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        __builtin_bit_cast(i32x8, av[s] & keep_first), bv, acc[s][e], 0, 0, 0, sa[s], 0, sbe);
+                        __builtin_bit_cast(i32x8, kSplitA ? av[s] : av[s] & keep_first), bv, acc[s][e], 0, 0, 0,
+                        sa[s], 0, sbe);
             }
             // The empty statements use each result where it is made.  Without
             // them the compiler sinks these MFMAs into the blocks that follow
@@ -331,7 +358,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
             // then lives on beside the next one, and an accumulator whose old
             // value is still live gets a second register set (the MFMA form
             // whose result may not overlap its input) -- over 210 VGPRs and 2
-            // waves per SIMD for one row set at 128 columns, against 155 and 3.
+            // waves per SIMD for one row set at 128 columns, against 153 and 3.
 #pragma unroll
             for (int k = 0; k < HB; ++k) {
                 const int e = half * HB + k;
@@ -348,7 +375,8 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
                     for (int s = 0; s < RS; ++s)
                         acc[s][e] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                            __builtin_bit_cast(i32x8, av[s] & ~keep_first), bv, acc[s][e], 0, 0, 0, sa[s], 0, sbo);
+                            __builtin_bit_cast(i32x8, kSplitA ? ao[s] : av[s] & ~keep_first), bv, acc[s][e], 0,
+                            0, 0, sa[s], 0, sbo);
                 }
 #pragma unroll
                 for (int k = 0; k < HB; ++k) {
@@ -367,15 +395,18 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         // The octet loop.  For a chunk whose slab scales differ, each octet's
         // A load also brings the octet's B scales.
         auto octets = [&]() {
-            auto next_a = [&](int q, u32x8 (&av)[RS], int (&sa)[RS], int &sbe, int &sbo) {
+            auto next_a = [&](int q, u32x8 (&av)[RS], u32x8 (&ao)[RS], int (&sa)[RS], int &sbe, int &sbo) {
                 load_a(q, av, sa);
-                if (mixed) load_bscale(q, sbe, sbo);
+                if (mixed) {
+                    if constexpr (kSplitA) load_a_odd(q, ao);
+                    load_bscale(q, sbe, sbo);
+                }
             };
             auto half_x = [&](int half, const u32x8 (&buf)[HB], bool second) {
                 if (second)
-                    compute_half(half, buf, avB, saB, sbeB, sboB);
+                    compute_half(half, buf, avB, aoB, saB, sbeB, sboB);
                 else
-                    compute_half(half, buf, avA, saA, sbeA, sboA);
+                    compute_half(half, buf, avA, aoA, saA, sbeA, sboA);
             };
             if constexpr (RS == 1 && EPL >= 4) {
                 // The loop runs while the row has two more octets after the
@@ -390,7 +421,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(0, bufX, false);
                     __builtin_amdgcn_sched_barrier(0);
-                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    next_a(q + 1, avB, aoB, saB, sbeB, sboB);
                     issue_half(lo_n, hi_n, 0, bufX);
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(1, bufY, false);
@@ -401,7 +432,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(0, bufX, true);
                     __builtin_amdgcn_sched_barrier(0);
-                    next_a(q + 2, avA, saA, sbeA, sboA);
+                    next_a(q + 2, avA, aoA, saA, sbeA, sboA);
                     issue_half(lo_n, hi_n, 0, bufX);
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(1, bufY, true);
@@ -415,7 +446,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                 half_x(0, bufX, false);
                 __builtin_amdgcn_sched_barrier(0);
                 if (q + 1 < q1) {
-                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    next_a(q + 1, avB, aoB, saB, sbeB, sboB);
                     issue_half(lo_n, hi_n, 0, bufX);
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(1, bufY, false);
@@ -436,7 +467,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(0, bufX, false);
                     __builtin_amdgcn_sched_barrier(0);
-                    next_a(q + 1, avB, saB, sbeB, sboB);
+                    next_a(q + 1, avB, aoB, saB, sbeB, sboB);
                     issue_half(lo_n, hi_n, 0, bufX);
                     __builtin_amdgcn_sched_barrier(0);
                     half_x(1, bufY, false);
@@ -447,7 +478,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
                     __builtin_amdgcn_sched_barrier(0);
                     if (q + 1 < q1) half_x(0, bufX, true);
                     __builtin_amdgcn_sched_barrier(0);
-                    next_a(q + 2, avA, saA, sbeA, sboA);
+                    next_a(q + 2, avA, aoA, saA, sbeA, sboA);
                     issue_half(lo_n, hi_n, 0, bufX);
                     __builtin_amdgcn_sched_barrier(0);
                     if (q + 1 < q1) half_x(1, bufY, true);
@@ -461,8 +492,16 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
         // checked after the pipeline's first loads are issued
         sb = scale_row_uniform(a, n_base / kChunk, srow);
         mixed = sb < 0;
-        keep_first = mixed ? keep_even : ~0u;
         if (mixed) {
+            keep_first = keep_even;
+            // the first octet's A came in whole: split it by lane group
+            if constexpr (kSplitA) {
+#pragma unroll
+                for (int s = 0; s < RS; ++s) {
+                    aoA[s] = avA[s] & ~keep_even;
+                    avA[s] = avA[s] & keep_even;
+                }
+            }
             load_bscale(q0, sbeA, sboA);
         } else {
             sbeA = sbeB = sb;

@@ -66,7 +66,7 @@ __device__ __forceinline__ void run_rows(const SpmmArgs &a, int tile, int s0, in
         for (int s = s0; s < s1; ++s) {
             const int br = a.row_order[s];
             if constexpr (PREC == BPREC_FP8)
-                fp8dev::fp8_work<RS, EPL>(a, br, tile);
+                fp8dev::fp8_work<RS, EPL, false>(a, br, tile);
             else
                 pairdev::pair_work<RS, EPL, batch_prefetch(PREC, RS, EPL), PREC>(a, br, tile);
         }
