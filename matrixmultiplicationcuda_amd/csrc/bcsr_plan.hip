@@ -435,10 +435,12 @@ int resolve_launch(const bcsr_plan *plan, int N, const bcsr_launch_cfg *cfg, Lau
         if (fp8) {
             // FP8, 32-row blocks at N = 128: 64-column tiles (18.1 vs 18.6 us
             // at 5 %, 28.3 vs 31.4 at 20 %); 16/32-row blocks in the range
-            // above with >= 15 % blocks: 256-column tiles (20 %, N = 512:
-            // 84.6 vs 90.6 us for 16-row blocks, 59.1 vs 63.5 for 32-row)
+            // above with >= 8 % blocks: 256-column tiles (whole calls at
+            // 16384^2, 16-row blocks: 8 %, N = 512: 44.9 vs 46.9 us; 12 %:
+            // 55.8 vs 60.7; N = 1024, 10 %: 86.4 vs 94.9; 32-row blocks,
+            // N = 512, 10 %: 38.9 vs 41.3; at 5 % the widths are level)
             if (plan->rs == 2 && N > 64 && N <= 128) tile_n = 64;
-            if (wide_ok && density >= 0.15) tile_n = 256;
+            if (wide_ok && density >= 0.08) tile_n = 256;
         }
         // Products with few work items (device block-rows x column tiles
         // under kFewItems, about 2.5 per CU) leave CUs idle at these widths:
