@@ -217,6 +217,30 @@ __device__ __forceinline__ bool map_work(const SpmmArgs &a, int &tile, int &slot
 // soffset strides for loads already known to be in bounds.
 constexpr unsigned kOutOfRange = 0x80000000u;
 
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+// p[i] read through the constant address space, so that a wave-uniform
+// address always becomes a scalar load.  The plan's row pointers, block
+// columns and A scales are written before any launch and never by a kernel;
+// read as generic pointers, the compiler proves that only while the kernel
+// holds no store it cannot place, and otherwise turns these reads into vector
+// loads that wait for every load in flight.
+template <typename T>
+__device__ __forceinline__ T const_load(const T *p, size_t i) {
+    return reinterpret_cast<const __attribute__((address_space(4))) T *>(reinterpret_cast<uintptr_t>(p))[i];
+}
+
+// p as a global-memory pointer.  A pointer that a kernel reads from memory
+// (the batched kernel's per-problem arguments) is generic to the compiler,
+// and its loads become flat loads, which count on both the vector-memory and
+// the scalar-memory counters: waiting for one then waits for every load in
+// flight.  Global loads count on the vector-memory counter only.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const T *p) {
+    return reinterpret_cast<const __attribute__((address_space(1))) T *>(reinterpret_cast<uintptr_t>(p));
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
     // gfx9 buffer descriptor word 3; num_records = bytes (reads past it return 0)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), static_cast<short>(0), bytes, 0x00020000);

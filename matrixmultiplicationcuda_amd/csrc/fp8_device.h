@@ -33,19 +33,6 @@ constexpr int kChunk = 256;  // columns per B scale
 // Bytes from one chunk's slab scales to the next (16-byte aligned rows).
 __host__ __device__ constexpr int scale_stride(int slabs) { return (slabs + 15) & ~15; }
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-
-// p[i] read through the constant address space, so that a wave-uniform
-// address always becomes a scalar load.  The plan's row pointers, octet
-// columns and A scales are written before any launch and never by a kernel;
-// read as generic pointers, the compiler proves that only while the kernel
-// holds no store it cannot place, and otherwise turns these reads into vector
-// loads that wait for every load in flight (a vmcnt(0) in the octet loop).
-template <typename T>
-__device__ __forceinline__ T const_load(const T *p, size_t i) {
-    return reinterpret_cast<const __attribute__((address_space(4))) T *>(reinterpret_cast<uintptr_t>(p))[i];
-}
-
 // Smallest power of two 2^k (k in [-127, 127]) with absmax / 2^k <= 448, as
 // the E8M0 byte k + 127; 127 for an all-zero group.
 __device__ __forceinline__ unsigned e8m0_for_absmax(float amax) {

@@ -202,13 +202,15 @@ __device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile
 #pragma unroll
             for (int e = 0; e < EPL; ++e) acc.v[h][s][e] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int p0 = a.grp_ptr[br], p1 = a.grp_ptr[br + 1];
+    const int p0 = const_load(a.grp_ptr, br), p1 = const_load(a.grp_ptr, br + 1);
     if (p0 < p1) {
-        const u32x4 *vals = a.vals + lane;
-        const int2 *cols = reinterpret_cast<const int2 *>(a.grp_col);
+        // global (not generic) loads for A, scalar ones for the columns, also
+        // where the pointers come from memory (the batched kernel)
+        const auto *vals = global_ptr(a.vals + lane);
+        const i32x2 *cols = reinterpret_cast<const i32x2 *>(a.grp_col);
         // Loads of pair p with columns cc (p >= p1: an out-of-range B offset
         // and the last pair's A).
-        auto issue = [&](int p, int2 cc, BRow<EPL>(&raw)[8], AOps<RS, PREC> &av) {
+        auto issue = [&](int p, i32x2 cc, BRow<EPL>(&raw)[8], AOps<RS, PREC> &av) {
             const int pc = p < p1 ? p : p1 - 1;
             const unsigned off =
                 p < p1 ? static_cast<unsigned>(sel_hi ? cc.y : cc.x) * slab_bytes + lane_off : kOutOfRange;
@@ -218,9 +220,9 @@ __device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile
 #pragma unroll
                 for (int h = 0; h < H; ++h) av.v[s][h] = vals[((static_cast<size_t>(pc) * RS + s) * H + h) * 64];
         };
-        auto col_of = [&](int p) { return cols[p < p1 ? p : p1 - 1]; };
+        auto col_of = [&](int p) { return const_load(cols, p < p1 ? p : p1 - 1); };
         if constexpr (!PF) {
-            int2 cn = col_of(p0);
+            i32x2 cn = col_of(p0);
             for (int p = p0; p < p1; ++p) {
                 BRow<EPL> raw[8];
                 AOps<RS, PREC> av;
@@ -231,7 +233,7 @@ __device__ __forceinline__ void pair_work(const SpmmArgs &a, int br_in, int tile
         } else {
             BRow<EPL> rawA[8], rawB[8];
             AOps<RS, PREC> avA, avB;
-            int2 cn = col_of(p0);
+            i32x2 cn = col_of(p0);
             issue(p0, cn, rawA, avA);
             cn = col_of(p0 + 1);
             // sched_barrier keeps the compiler from sinking a pair's loads
