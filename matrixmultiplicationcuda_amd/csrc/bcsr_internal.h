@@ -240,6 +240,12 @@ template <typename T>
 __device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const T *p) {
     return reinterpret_cast<const __attribute__((address_space(1))) T *>(reinterpret_cast<uintptr_t>(p));
 }
+// (stores: a flat store also counts on both counters, so the next scalar
+// load's wait waits for the stores in flight)
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *global_ptr(T *p) {
+    return reinterpret_cast<__attribute__((address_space(1))) T *>(reinterpret_cast<uintptr_t>(p));
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes) {
     // gfx9 buffer descriptor word 3; num_records = bytes (reads past it return 0)

@@ -67,8 +67,8 @@ __device__ __forceinline__ void load_slab_part(const _Float16 *__restrict__ B, i
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const bool ok = sp.ok && (u == 0 || sp.ok1);
-            sp.v[u][k] = ok ? *reinterpret_cast<const u32x4 *>(
-                                  B + static_cast<size_t>((2 * x + u) * 16 + 4 * sp.m + k) * N + sp.n)
+            sp.v[u][k] = ok ? *global_ptr(reinterpret_cast<const u32x4 *>(
+                                  B + static_cast<size_t>((2 * x + u) * 16 + 4 * sp.m + k) * N + sp.n))
                             : u32x4{0u, 0u, 0u, 0u};
         }
 }
@@ -114,7 +114,7 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, uint8
         m = block_max128(m, wmax);
         const unsigned se = e8m0_for_absmax(m);
         if (threadIdx.x == 0 && (u == 0 || sp.ok1))
-            scale[static_cast<size_t>(h) * scale_stride(slabs) + 2 * x + u] = static_cast<uint8_t>(se);
+            global_ptr(scale)[static_cast<size_t>(h) * scale_stride(slabs) + 2 * x + u] = static_cast<uint8_t>(se);
         inv[u] = ldexpf(1.0f, 127 - static_cast<int>(se));
     }
     const int g8 = threadIdx.x >> 2;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void quant_slab(const _Float16 *__restrict__ B, uint8
             const int p = col ^ ((col >> 3) & 7);
             const int n = h * kChunk + col;
             if (n < N)
-                *reinterpret_cast<u32x4 *>(B8 + (static_cast<size_t>(2 * x + u) * N + n) * 16) =
+                *global_ptr(reinterpret_cast<u32x4 *>(B8 + (static_cast<size_t>(2 * x + u) * N + n) * 16)) =
                     u32x4{rec[u][4 * p], rec[u][4 * p + 1], rec[u][4 * p + 2], rec[u][4 * p + 3]};
         }
     }
@@ -505,7 +505,7 @@ __device__ __forceinline__ void fp8_work(const SpmmArgs &a, int br_in, int tile_
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const size_t row = static_cast<size_t>(br) * (16 * RS) + 16 * s + 4 * g + r;
-                a.C[row * static_cast<size_t>(a.N) + n] = acc[s][e][r];
+                global_ptr(a.C)[row * static_cast<size_t>(a.N) + n] = acc[s][e][r];
             }
     }
 }

@@ -64,7 +64,7 @@ template <int PREC, int RS, int EPL>
 __device__ __forceinline__ void run_rows(const SpmmArgs &a, int tile, int s0, int s1) {
     if constexpr (batch_shape(PREC, RS, EPL)) {
         for (int s = s0; s < s1; ++s) {
-            const int br = a.row_order[s];
+            const int br = const_load(a.row_order, s);  // a scalar load (a's pointers come from memory)
             if constexpr (PREC == BPREC_FP8)
                 fp8dev::fp8_work<RS, EPL, false>(a, br, tile);
             else

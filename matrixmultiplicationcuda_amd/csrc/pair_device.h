@@ -146,7 +146,7 @@ __device__ __forceinline__ void store_tile(const Acc<RS, EPL, PREC> &acc, float 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const size_t row = static_cast<size_t>(br) * (16 * RS) + 16 * s + 4 * g + r;
-            float *cp = C + row * N + n0;
+            auto *cp = global_ptr(C + row * N + n0);
             float o[EPL];
 #pragma unroll
             for (int e = 0; e < EPL; ++e) {
@@ -156,11 +156,12 @@ __device__ __forceinline__ void store_tile(const Acc<RS, EPL, PREC> &acc, float 
                     o[e] = acc.v[0][s][e][r];
             }
             if constexpr (EPL == 2) {
-                *reinterpret_cast<f32x2 *>(cp) = f32x2{o[0], o[1]};
+                *reinterpret_cast<__attribute__((address_space(1))) f32x2 *>(cp) = f32x2{o[0], o[1]};
             } else {
 #pragma unroll
                 for (int h = 0; h < EPL / 4; ++h)
-                    reinterpret_cast<f32x4 *>(cp)[h] = f32x4{o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]};
+                    reinterpret_cast<__attribute__((address_space(1))) f32x4 *>(cp)[h] =
+                        f32x4{o[4 * h], o[4 * h + 1], o[4 * h + 2], o[4 * h + 3]};
             }
         }
 }
