@@ -171,7 +171,7 @@ int bcsr_spmm_kernel_name(const bcsr_plan *plan, int32_t N, const bcsr_launch_cf
  * workspaces), so they must be ordered (one stream, or events). */
 typedef struct bcsr_batch bcsr_batch;
 typedef struct {
-    int32_t waves_per_cu; /* persistent waves per CU: 0 = 8 */
+    int32_t waves_per_cu; /* persistent waves per CU: 0 = 8 (FP8) or 6 (16-bit modes) */
     int32_t bin_work;     /* work per queue item, in 16 x 16 x 32 MFMA-tile steps: 0 = auto */
     int32_t queues;       /* 0 = auto, 1 = one global queue, 2 = one queue per XCD (column tiles
                            * pinned to XCDs, idle waves take the other queues' items) */

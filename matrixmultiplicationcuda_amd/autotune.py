@@ -28,7 +28,7 @@ XCD_MAPS = (0, 1, 2, 3, 4)  # linear, tiles pinned, rows pinned, chunked runs, 2
 # (say tiles pinned to XCDs for a tile count 8 does not divide) is missing
 CONFIGS_16 = tuple((t, v, m) for t in TILES for v in (0, 1) for m in XCD_MAPS)
 CONFIGS_FP8 = tuple((t, -1, m) for t in TILES for m in XCD_MAPS)
-BATCH_WAVES = (4, 8, 12, 16)
+BATCH_WAVES = (4, 6, 8, 12, 16)
 BATCH_BINS = (32, 64, 128, 256)
 
 

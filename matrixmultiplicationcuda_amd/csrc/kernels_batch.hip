@@ -224,12 +224,16 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
     int n_cu = 0;
     rc = hip_status(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     if (rc != BCSR_OK) return rc;
-    const int wpc = cfg && cfg->waves_per_cu > 0 ? cfg->waves_per_cu : 8;
+    const int precision = count ? plans[0]->info.precision : BCSR_PREC_F16;
+    const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
+    // defaults, fastest over the 96-problem batched workload on MI355X (graph
+    // replays): FP8 8 waves per CU and items of up to 64 units (0.434 ms;
+    // 6 and 64: 0.433); the 16-bit modes 6 and up to 128 (fp16 0.489 against
+    // 0.526 ms at 8 and 64, bf16 0.492 against 0.521)
+    const int wpc = cfg && cfg->waves_per_cu > 0 ? cfg->waves_per_cu : (fp8 ? 8 : 6);
     if (wpc > 32) return BCSR_ERR_INVALID_ARG;
     const int qmode = cfg ? cfg->queues : 0;
     if (qmode < 0 || qmode > 2) return BCSR_ERR_INVALID_ARG;
-    const int precision = count ? plans[0]->info.precision : BCSR_PREC_F16;
-    const bool fp8 = precision == BCSR_PREC_FP8_E4M3;
 
     std::vector<BatchMat> mats(count);
     std::vector<int32_t> tiles(count);
@@ -308,9 +312,10 @@ static int batch_create(const bcsr_plan *const *plans, const void *const *B, flo
             total_work += (p->h_grp_ptr[r + 1] - p->h_grp_ptr[r] + 1.0) * p->rs * (tiles[i] / 16) * nt;
     }
     const int grid = n_cu * wpc;
-    // default: items of 64 units (fastest measured over the 96-problem
-    // batched workload on MI355X), fewer only when the batch is small
-    double bin = cfg && cfg->bin_work > 0 ? cfg->bin_work : std::clamp(total_work / (4.0 * grid), 8.0, 64.0);
+    // default item size: the cap above, fewer units only when the batch is
+    // small (at least four items per wave)
+    const double bin_cap = fp8 ? 64.0 : 128.0;
+    double bin = cfg && cfg->bin_work > 0 ? cfg->bin_work : std::clamp(total_work / (4.0 * grid), 8.0, bin_cap);
     for (int i = 0; i < count; ++i) {
         const bcsr_plan *p = plans[i];
         const int nt = mats[i].a.n_tiles;
