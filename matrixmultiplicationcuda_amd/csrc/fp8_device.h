@@ -14,11 +14,12 @@
 // so slots (2 t, 2 t + 1) form scale group t and share one A scale (the
 // scale of a 16 x 32 tile).  B is quantised per call in one pass, with one
 // E8M0 scale per (16-row slab, 256-column chunk), stored chunk-major with
-// rows padded to 16 bytes: bscale[chunk * scale_stride(slabs) + slab].  Where every slab of a chunk has the same
-// scale (checked per row at its start), any two slabs can share a scale group
-// and an octet is one MFMA per 16 columns; otherwise each octet takes two,
-// one over its even slots (A's odd lane groups zeroed) with the even slabs'
-// scales and one over its odd slots with theirs.  B is stored k-minor,
+// rows padded to 16 bytes: bscale[chunk * scale_stride(slabs) + slab].
+// Where every slab of a chunk has the same scale (checked per row at its
+// start), any two slabs can share a scale group and an octet is one MFMA per
+// 16 columns; otherwise each octet takes two, one over its even slots (A's
+// odd lane groups zero) with the even slabs' scales and one over its odd
+// slots with theirs, in the same pipelined loop.  B is stored k-minor,
 // B8[slab][n][16]: the 16 rows of slab c for column n are 16 consecutive
 // bytes, exactly one lane's half operand.  All 64 lanes carry data: one octet
 // step moves 8 blocks through one MFMA per 16 columns.
@@ -75,7 +76,8 @@ __device__ __forceinline__ void load_slab_part(const _Float16 *__restrict__ B, i
 
 __device__ __forceinline__ float half_at(const u32x4 &v, int e) {
     const unsigned d = v[e >> 1];
-    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>((e & 1) ? d >> 16 : d & 0xffffu)));
+    const auto bits = static_cast<unsigned short>((e & 1) ? d >> 16 : d & 0xffffu);
+    return static_cast<float>(__builtin_bit_cast(_Float16, bits));
 }
 
 // Maximum over a 64-lane wave, then over the two waves of the workgroup
